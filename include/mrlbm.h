@@ -1,0 +1,162 @@
+/*
+ * mrlbm.h -- C-ABI of the MI355X (gfx950) D2Q9 moments-based regularized LBM time step.
+ *
+ * The reference (sakthi-lbm/mrlbm_2d) has no callable interface: its time step is the
+ * inline body of `program lbm_2d` (2d_mrlbm.f90:229-555) talking through program-scope
+ * arrays (2d_mrlbm.f90:3-73).  This header is the boundary placed at that loop body.
+ * Each entry point names the reference code it replaces.  All arrays are host arrays in
+ * the reference's column-major order (x fastest; field(nx,ny), f(nx,ny,0:8)); the
+ * library copies synchronously and keeps no host pointers.  Every call returns 0 on
+ * success or a negative MRLBM_E_* code; mrlbm_last_error() explains the last failure.
+ * One host thread per handle; calls on one handle are not re-entrant.
+ */
+#ifndef MRLBM_H
+#define MRLBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRLBM_ABI_VERSION 1
+
+enum {
+    MRLBM_OK = 0,
+    MRLBM_E_ARG = -1,        /* bad argument or unsupported configuration */
+    MRLBM_E_HIP = -2,        /* HIP runtime failure */
+    MRLBM_E_GEOMETRY = -3,   /* cylinder too close to an edge, too large, or unsupported label */
+    MRLBM_E_NONFINITE = -4,  /* NaN/Inf detected by mrlbm_check_finite */
+    MRLBM_E_NOMEM = -5,
+    MRLBM_E_STATE = -6       /* call out of order (e.g. nothing recorded) */
+};
+
+enum { MRLBM_FP64 = 0, MRLBM_FP32_DEV = 1 };                        /* precision form */
+enum { MRLBM_MASS_STRONG = 0, MRLBM_MASS_EQ = 1, MRLBM_MASS_WEAK = 2 }; /* curved-wall density rule */
+enum { MRLBM_INLET_UNIFORM = 0, MRLBM_INLET_PARABOLIC = 1 };
+enum { MRLBM_LABEL_REFERENCE = 0, MRLBM_LABEL_FLUID = 1 };          /* policy for labels 0/6/9/15 */
+
+/* Run configuration.  Replaces the namelist-derived globals of 2d_mrlbm.f90:83-178
+ * (&DomainSize/&Cylinder/&Numbers/&Controls/&LogicalControls, read at :2222-2275). */
+typedef struct mrlbm_params {
+    int32_t nx, ny;               /* lattice size (2d_mrlbm.f90:86-92 derives it from ncy, L_*) */
+    int32_t x_periodic, y_periodic;
+    int32_t channel_with_cylinder;
+    double x_c, y_c, r_c;         /* 1-based lattice coordinates of the centre, nominal radius
+                                     (2d_mrlbm.f90:946-948) */
+    double Re;                    /* nu = uo*2*r_cyl/Re, or uo*(ny-1)/Re without cylinder (:133-134) */
+    double uo;                    /* inlet axial velocity uinlet (:123) */
+    double omega;                 /* > 0 overrides the value derived from Re */
+    int32_t inlet_profile;        /* MRLBM_INLET_*; uniform is the reference (:258) */
+    int32_t mass_bc;              /* MRLBM_MASS_*: strong (:1635), eq (:1873), weak (:1767) */
+    int32_t rotated_coordinate;   /* 0: unrotated BC with Cramer solve (:1496-1633, :1974) */
+    int32_t vel_interp, mom_interp;
+    int32_t incomp_bc, incomp_bulk;
+    int32_t precision;            /* MRLBM_FP64 or MRLBM_FP32_DEV */
+    int32_t label_policy;         /* MRLBM_LABEL_REFERENCE reproduces the reference (NaN on
+                                     labels the reference leaves unset, :2008) */
+    int32_t strip_rows;           /* tuning: rows per bulk strip (0 = auto) */
+    int32_t kernel;               /* 0: row-marching bulk kernel; 1: one node per thread (checking) */
+    int32_t reserved[6];
+} mrlbm_params;
+
+/* What one call to mrlbm_step records (2d_mrlbm.f90:355-502). */
+typedef struct mrlbm_step_opts {
+    int32_t post_process;         /* &LogicalControls post_process */
+    int32_t statsbegin, statsend; /* window for forces, surface p, inlet-rho mean (:357, :467) */
+    int32_t iplot;                /* solid-node u is zeroed before collision on iter%iplot==0 (:371-384) */
+    int32_t record_surface_p;     /* compute the surface-pressure feed each window step (:479) */
+    int32_t stats_interval;       /* >0: accumulate mean/RMS u fields every n window steps */
+    int32_t nprobe;               /* 0..4 probes (1-based i,j) recorded every window step (:471-476) */
+    int32_t probe_i[4], probe_j[4];
+    int32_t reserved[6];
+} mrlbm_step_opts;
+
+typedef struct mrlbm_info {
+    int32_t nx, ny, nbct, precision;
+    double r_cyl;                 /* largest boundary-node radius (2d_mrlbm.f90:1038-1050) */
+    double nu, omega, x_c, y_c, r_c;
+    int64_t bytes_per_step;       /* algorithmic HBM bytes of one bulk step (12 words/node) */
+    int32_t box[4];               /* 0-based x0, y0, width, height of the cylinder box */
+    int32_t n_stencil;            /* unique stencil nodes of the boundary workgroup */
+    int32_t levels;               /* dependency levels of the in-place velocity interpolation */
+} mrlbm_info;
+
+typedef struct mrlbm_handle mrlbm_handle;
+
+/* Setup: replaces allocate_memory/get_coord/initialize and the rotated tables
+ * (2d_mrlbm.f90:94-214, 931-1235, 2005-2057, 2277-2332).  The state starts at rest
+ * (rho = 1, u = 0: f = w_k, 2d_mrlbm.f90:1220-1235). */
+int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out);
+int mrlbm_destroy(mrlbm_handle *h);
+/* Last error message of h, or of the last failed mrlbm_create when h is NULL. */
+const char *mrlbm_last_error(const mrlbm_handle *h);
+int mrlbm_abi_version(void);
+void mrlbm_default_params(mrlbm_params *p);
+void mrlbm_default_step_opts(mrlbm_step_opts *o);
+
+int mrlbm_get_info(const mrlbm_handle *h, mrlbm_info *out);
+/* Boundary list in the reference's order (i outer, j inner; 2d_mrlbm.f90:1007-1033):
+ * 1-based bou_i/bou_j, quadrant label, theta (radians in [0,2pi), :1132-1133) and the
+ * theta-sorted permutation idx (1-based, :1137-1152).  Any pointer may be NULL. */
+int mrlbm_get_boundary(const mrlbm_handle *h, int32_t *bou_i, int32_t *bou_j, int32_t *label,
+                       double *theta, int32_t *idx);
+/* Node classes (1 = fluid, 2 = solid, 3 = boundary) as int8, (nx,ny) column-major. */
+int mrlbm_get_node_class(const mrlbm_handle *h, int8_t *cls);
+
+/* State in: post-stream populations f(nx,ny,0:8), as read from restart.dat
+ * (read_restart_file, 2d_mrlbm.f90:2204-2220).  The next step starts from exactly these
+ * populations, and out-of-domain (stale) populations keep their values from f. */
+int mrlbm_set_populations(mrlbm_handle *h, const double *f);
+/* State in: post-collision moments; f_k = reconstruction of (rho, u, m) (2d_mrlbm.f90:445-457).
+ * mxx/myy/mxy may be NULL for u_a*u_b (the second-order equilibrium). */
+int mrlbm_set_moments(mrlbm_handle *h, const double *rho, const double *ux, const double *uy,
+                      const double *mxx, const double *myy, const double *mxy);
+
+/* Advance nsteps iterations numbered iter_first..iter_first+nsteps-1 (the body of the
+ * main loop, 2d_mrlbm.f90:229-535).  opts may be NULL (no recording). */
+int mrlbm_step(mrlbm_handle *h, int32_t iter_first, int32_t nsteps, const mrlbm_step_opts *opts);
+int mrlbm_synchronize(mrlbm_handle *h);
+
+/* Post-BC rho, ux, uy of the last step (the arrays written to plot3d and restart,
+ * 2d_mrlbm.f90:2178-2180, :2198).  Any pointer may be NULL. */
+int mrlbm_get_fields(const mrlbm_handle *h, double *rho, double *ux, double *uy);
+/* All six post-collision moments (rho, ux, uy, mxx, myy, mxy). */
+int mrlbm_get_moments(const mrlbm_handle *h, double *rho, double *ux, double *uy,
+                      double *mxx, double *myy, double *mxy);
+/* Post-stream populations f(nx,ny,0:8) for restart.dat (write_restart_file, :2186-2202). */
+int mrlbm_get_populations(const mrlbm_handle *h, double *f);
+
+/* Force series recorded since the last call (evaluate_forces + sums, :481-494):
+ * up to max entries of (iter, Fd, Fl); *count gets the number returned. */
+int mrlbm_get_forces(mrlbm_handle *h, int32_t *iters, double *fd, double *fl, int32_t max,
+                     int32_t *count);
+/* Surface pressure of the last recorded step, theta-sorted, raw and smoothed
+ * (write_statistics_out, :593-617).  Arrays of nbct. */
+int mrlbm_get_surface_p(const mrlbm_handle *h, int32_t *iter, double *ps, double *pfit);
+/* Surface-pressure series recorded since the last call: iters[n], ps[n*nbct], pfit[n*nbct]. */
+int mrlbm_get_surface_p_series(mrlbm_handle *h, int32_t *iters, double *ps, double *pfit, int32_t max,
+                               int32_t *count);
+/* Running mean of the inlet density (:357-360). */
+int mrlbm_get_rho_inf_mean(const mrlbm_handle *h, double *value);
+/* Probe series since the last call: per entry iter, then rho, ux, uy at each probe. */
+int mrlbm_get_probes(mrlbm_handle *h, int32_t *iters, double *vals, int32_t max, int32_t *count);
+/* Time-averaged fields (new in this library; the reference's iplotstats is unused, :6):
+ * mean ux, uy and rms of their fluctuations over the samples taken so far. */
+int mrlbm_get_stats(const mrlbm_handle *h, double *mean_ux, double *mean_uy, double *rms_ux,
+                    double *rms_uy, int64_t *nsamples);
+int mrlbm_reset_stats(mrlbm_handle *h);
+/* max(ux) over all nodes including solid ones, as logged every 100 steps (:546). */
+int mrlbm_max_ux(const mrlbm_handle *h, double *value);
+/* MRLBM_E_NONFINITE when any stored moment is NaN/Inf (the reference never checks). */
+int mrlbm_check_finite(const mrlbm_handle *h);
+
+/* Mean kernel time (ms) of the bulk step over the last mrlbm_step call, from HIP events
+ * recorded on the handle's stream; 0 when timing is off. */
+int mrlbm_set_timing(mrlbm_handle *h, int32_t on);
+int mrlbm_last_step_ms(const mrlbm_handle *h, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,803 @@
+// mrlbm_capi.hip -- the C-ABI of include/mrlbm.h: handles, device memory, the step loop.
+#include "mrlbm.h"
+#include "mrlbm_geometry.h"
+#include "mrlbm_kernels.h"
+#include "mrlbm_lattice.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace mrlbm;
+
+namespace {
+thread_local std::string g_create_error;
+
+template <class T> T *dmalloc(size_t n, hipError_t &e)
+{
+    void *p = nullptr;
+    e = hipMalloc(&p, n ? n * sizeof(T) : sizeof(T));
+    return (T *)p;
+}
+} // namespace
+
+struct mrlbm_handle {
+    mrlbm_params p{};
+    std::string err;
+    int nx = 0, ny = 0, pitch = 0;
+    long long plane = 0;
+    bool f32 = false;
+    Geometry geo;
+    BoundaryTables bt;
+    bool has_b = false;
+    double nu = 0, omega = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool timing = false;
+    double last_ms = 0;
+    void *S[2] = {nullptr, nullptr};
+    int cur = 0;
+    void *pops = nullptr;
+    bool from_pops = false;
+    void *ghost = nullptr;
+    void *uin = nullptr;
+    uint8_t *d_box = nullptr;
+    std::vector<void *> allocs; // boundary tables and scratch
+    BoundaryDev bd{};
+    double *d_rec = nullptr;
+    int rec_cap = 0, rec_stride = 0;
+    double *d_inlet[2] = {nullptr, nullptr};
+    double *d_rho_mean = nullptr;
+    double *d_acc = nullptr;
+    int64_t nsamples = 0;
+    double *d_red = nullptr; // reduction scratch
+    int prev_inlet_valid = 0, prev_inlet_buf = 0;
+    double prev_inlet_c = 0;
+    int strip_rows = 32, ncg = 0, ncb = 0, nstrips = 0;
+    // host-side queues of recorded series
+    std::vector<int32_t> q_force_iter;
+    std::vector<double> q_fd, q_fl;
+    std::vector<int32_t> q_probe_iter;
+    std::vector<double> q_probe;
+    std::vector<int32_t> q_surf_iter;
+    std::vector<double> q_ps, q_pfit;
+    int last_p_iter = -1;
+    std::vector<double> last_ps, last_pfit;
+    int last_nprobe = 0;
+};
+
+#define FAIL(code, msg)            \
+    do {                           \
+        h->err = (msg);            \
+        return (code);             \
+    } while (0)
+#define HIPCHK(...)                                                                     \
+    do {                                                                                \
+        hipError_t e_ = (__VA_ARGS__);                                                  \
+        if (e_ != hipSuccess) FAIL(MRLBM_E_HIP, std::string(#__VA_ARGS__) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" int mrlbm_abi_version(void) { return MRLBM_ABI_VERSION; }
+
+extern "C" void mrlbm_default_params(mrlbm_params *p)
+{
+    std::memset(p, 0, sizeof *p);
+    // defaults of the reference's input.dat (input.dat:1-45)
+    p->x_periodic = 0;
+    p->y_periodic = 1;
+    p->channel_with_cylinder = 1;
+    p->Re = 100.0;
+    p->uo = 0.1;
+    p->omega = 0.0;
+    p->inlet_profile = MRLBM_INLET_UNIFORM;
+    p->mass_bc = MRLBM_MASS_STRONG;
+    p->rotated_coordinate = 1;
+    p->vel_interp = 1;
+    p->mom_interp = 1;
+    p->incomp_bc = 0;
+    p->incomp_bulk = 0;
+    p->precision = MRLBM_FP64;
+    p->label_policy = MRLBM_LABEL_REFERENCE;
+}
+
+extern "C" void mrlbm_default_step_opts(mrlbm_step_opts *o)
+{
+    std::memset(o, 0, sizeof *o);
+    o->post_process = 0;
+    o->statsbegin = 1 << 30;
+    o->statsend = -1;
+    o->iplot = 0;
+}
+
+extern "C" const char *mrlbm_last_error(const mrlbm_handle *h)
+{
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+template <class S, class T> static StepArgs<S, T> base_args(const mrlbm_handle *h)
+{
+    StepArgs<S, T> a{};
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.pitch = h->pitch;
+    a.plane = h->plane;
+    a.xper = h->p.x_periodic;
+    a.yper = h->p.y_periodic;
+    a.src = (const S *)h->S[h->cur];
+    a.dst = (S *)h->S[h->cur ^ 1];
+    a.pops = h->from_pops ? (const T *)h->pops : nullptr;
+    a.ghost = (const T *)h->ghost;
+    a.om = (T)h->omega;
+    a.om1 = (T)(1.0 - h->omega);
+    a.vin = (T)0.0;
+    a.uin = (const T *)h->uin;
+    a.incomp_bc = h->p.incomp_bc;
+    a.incomp_bulk = h->p.incomp_bulk;
+    a.bx0 = h->bt.bx0;
+    a.by0 = h->bt.by0;
+    a.bw = h->bt.bw;
+    a.bh = h->bt.bh;
+    a.boxcls = h->d_box;
+    a.strip_rows = h->strip_rows;
+    a.ncg = h->ncg;
+    a.ncb = h->ncb;
+    a.nstrips = h->nstrips;
+    a.has_boundary = h->has_b ? 1 : 0;
+    a.simple = h->p.kernel == 1 ? 1 : 0;
+    a.rec_slot = -1;
+    a.rec = h->d_rec;
+    a.rec_stride = h->rec_stride;
+    a.acc = h->d_acc;
+    a.rho_inf_mean = h->d_rho_mean;
+    a.b = h->bd;
+    return a;
+}
+
+template <class T> static int upload(mrlbm_handle *h, void **dst, const std::vector<T> &v)
+{
+    hipError_t e;
+    T *p = dmalloc<T>(v.size(), e);
+    HIPCHK(e);
+    h->allocs.push_back(p);
+    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *dst = p;
+    return 0;
+}
+
+static int ensure_rec(mrlbm_handle *h)
+{
+    if (h->d_rec) return 0;
+    h->rec_stride = 14 + 2 * h->bt.nbct;
+    h->rec_cap = 256;
+    hipError_t e;
+    h->d_rec = dmalloc<double>((size_t)h->rec_cap * h->rec_stride, e);
+    HIPCHK(e);
+    HIPCHK(hipMemset(h->d_rec, 0, (size_t)h->rec_cap * h->rec_stride * sizeof(double)));
+    return 0;
+}
+
+template <class S, class T> static int init_rest_state(mrlbm_handle *h)
+{
+    // rho = rho0 = 1, u = 0 (2d_mrlbm.f90:199-204): stored moments (1 or 0, 0, 0, 0, 0, 0)
+    std::vector<S> row((size_t)h->plane, (S)0);
+    for (int b = 0; b < 2; b++) {
+        const S r0 = h->f32 ? (S)0 : (S)1;
+        std::vector<S> rp((size_t)h->plane, r0);
+        HIPCHK(hipMemcpy(h->S[b], rp.data(), h->plane * sizeof(S), hipMemcpyHostToDevice));
+        for (int q = 1; q < 6; q++)
+            HIPCHK(hipMemcpy((S *)h->S[b] + q * h->plane, row.data(), h->plane * sizeof(S), hipMemcpyHostToDevice));
+    }
+    std::vector<T> g((size_t)Q * 2 * (h->nx + h->ny));
+    for (size_t t = 0; t < g.size() / Q; t++)
+        for (int k = 0; k < Q; k++) g[t * Q + k] = h->f32 ? (T)0 : (T)wk<double>(k);
+    HIPCHK(hipMemcpy(h->ghost, g.data(), g.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <class S, class T> static int create_typed(mrlbm_handle *h)
+{
+    hipError_t e;
+    const size_t state_bytes = (size_t)6 * h->plane * sizeof(S);
+    for (int b = 0; b < 2; b++) {
+        HIPCHK(hipMalloc(&h->S[b], state_bytes));
+    }
+    HIPCHK(hipMalloc(&h->ghost, (size_t)Q * 2 * (h->nx + h->ny) * sizeof(T)));
+    // inlet profile (uniform uo in the reference, 2d_mrlbm.f90:258)
+    std::vector<T> uin(h->ny);
+    for (int j = 0; j < h->ny; j++) {
+        if (h->p.inlet_profile == MRLBM_INLET_PARABOLIC) {
+            const double s = (double)j / (double)(h->ny - 1);
+            uin[j] = (T)(4.0 * h->p.uo * s * (1.0 - s));
+        } else {
+            uin[j] = (T)h->p.uo;
+        }
+    }
+    HIPCHK(hipMalloc(&h->uin, h->ny * sizeof(T)));
+    HIPCHK(hipMemcpy(h->uin, uin.data(), h->ny * sizeof(T), hipMemcpyHostToDevice));
+    for (int b = 0; b < 2; b++) {
+        h->d_inlet[b] = dmalloc<double>(h->ny, e);
+        HIPCHK(e);
+    }
+    h->d_rho_mean = dmalloc<double>(1, e);
+    HIPCHK(e);
+    HIPCHK(hipMemset(h->d_rho_mean, 0, sizeof(double)));
+    h->d_red = dmalloc<double>(4096, e);
+    HIPCHK(e);
+    if (h->has_b) {
+        const BoundaryTables &t = h->bt;
+        BoundaryDev &b = h->bd;
+        b.nbct = t.nbct;
+        b.nU = t.nU;
+        b.nlevels = t.nlevels;
+        int rc = 0;
+        void *p;
+        rc |= upload(h, &p, t.node); b.node = (const int32_t *)p;
+        rc |= upload(h, &p, t.ij); b.ij = (const int32_t *)p;
+        rc |= upload(h, &p, t.label); b.label = (const int32_t *)p;
+        rc |= upload(h, &p, t.uself); b.uself = (const int32_t *)p;
+        rc |= upload(h, &p, t.uc); b.uc = (const int32_t *)p;
+        rc |= upload(h, &p, t.dep); b.dep = (const int32_t *)p;
+        rc |= upload(h, &p, t.level_order); b.level_order = (const int32_t *)p;
+        rc |= upload(h, &p, t.level_start); b.level_start = (const int32_t *)p;
+        rc |= upload(h, &p, t.real); b.real = (const double *)p;
+        rc |= upload(h, &p, t.unode); b.unode = (const int32_t *)p;
+        rc |= upload(h, &p, t.uij); b.uij = (const int32_t *)p;
+        rc |= upload(h, &p, t.urot); b.urot = (const double *)p;
+        rc |= upload(h, &p, t.idx); b.idx = (const int32_t *)p;
+        rc |= upload(h, &p, t.thetas); b.thetas = (const double *)p;
+        rc |= upload(h, &p, t.boxcls); h->d_box = (uint8_t *)p;
+        if (rc) return rc;
+        label_masks(h->p.label_policy, b.incs, b.outs);
+        for (int l = 0; l < 16; l++) { // rest-population part of the momentum exchange (FP32 form)
+            double fx = 0, fy = 0;
+            for (int k = 0; k < Q; k++) {
+                const double w = wk<double>(k);
+                if ((b.incs[l] >> k) & 1) { fx += w * EXk(k); fy += w * EYk(k); }
+                if ((b.outs[l] >> k) & 1) { fx -= w * EXk(k); fy -= w * EYk(k); }
+            }
+            b.wfx[l] = fx;
+            b.wfy[l] = fy;
+        }
+        b.mass_bc = h->p.mass_bc;
+        b.rotated = h->p.rotated_coordinate;
+        b.vel_interp = h->p.vel_interp;
+        b.mom_interp = h->p.mom_interp;
+        b.U = dmalloc<T>((size_t)5 * t.nU, e);
+        HIPCHK(e);
+        h->allocs.push_back(b.U);
+        HIPCHK(hipMemset(b.U, 0, (size_t)5 * t.nU * sizeof(T)));
+        b.ub = dmalloc<T>((size_t)2 * t.nbct, e);
+        HIPCHK(e);
+        h->allocs.push_back(b.ub);
+        b.pcy = dmalloc<double>(t.nbct, e);
+        HIPCHK(e);
+        h->allocs.push_back(b.pcy);
+        b.fxy = dmalloc<double>((size_t)2 * t.nbct, e);
+        HIPCHK(e);
+        h->allocs.push_back(b.fxy);
+    }
+    return init_rest_state<S, T>(h);
+}
+
+static void free_all(mrlbm_handle *h)
+{
+    for (void *p : {h->S[0], h->S[1], h->pops, h->ghost, h->uin, (void *)h->d_rec, (void *)h->d_inlet[0],
+                    (void *)h->d_inlet[1], (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red})
+        if (p) (void)hipFree(p);
+    for (void *p : h->allocs)
+        if (p) (void)hipFree(p);
+    h->allocs.clear();
+    if (h->e0) (void)hipEventDestroy(h->e0);
+    if (h->e1) (void)hipEventDestroy(h->e1);
+    if (h->st) (void)hipStreamDestroy(h->st);
+}
+
+extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
+{
+    g_create_error.clear();
+    if (!p || !out) {
+        g_create_error = "null argument";
+        return MRLBM_E_ARG;
+    }
+    *out = nullptr;
+    auto *h = new mrlbm_handle();
+    h->p = *p;
+    auto fail = [&](int code, const std::string &m) {
+        g_create_error = m.empty() ? h->err : m;
+        free_all(h);
+        delete h;
+        return code;
+    };
+    if (p->nx < 4 || p->ny < 4) return fail(MRLBM_E_ARG, "nx and ny must be >= 4");
+    if (p->precision != MRLBM_FP64 && p->precision != MRLBM_FP32_DEV) return fail(MRLBM_E_ARG, "unknown precision");
+    if (p->mass_bc < 0 || p->mass_bc > 2) return fail(MRLBM_E_ARG, "unknown mass_bc");
+    if ((long long)p->nx * p->ny > (1LL << 31) - 1) return fail(MRLBM_E_ARG, "grid too large for 32-bit node index");
+    h->nx = p->nx;
+    h->ny = p->ny;
+    h->pitch = (p->nx + 63) / 64 * 64;
+    h->plane = (long long)h->pitch * p->ny;
+    h->f32 = p->precision == MRLBM_FP32_DEV;
+    if (p->channel_with_cylinder) {
+        GeometryInput gi{p->nx, p->ny, p->x_c, p->y_c, p->r_c, p->label_policy};
+        std::string m = build_geometry(gi, h->geo);
+        if (!m.empty()) return fail(MRLBM_E_GEOMETRY, m);
+        m = build_tables(h->geo, h->pitch, h->bt);
+        if (!m.empty()) return fail(MRLBM_E_GEOMETRY, m);
+        h->has_b = h->bt.nbct > 0;
+    }
+    // viscosity and relaxation (2d_mrlbm.f90:133-135)
+    const double uref = p->inlet_profile == MRLBM_INLET_PARABOLIC ? (2.0 / 3.0) * p->uo : p->uo;
+    if (p->channel_with_cylinder) h->nu = uref * (2.0 * h->geo.r_cyl) / p->Re;
+    else h->nu = uref * (double)(p->ny - 1) / p->Re;
+    h->omega = p->omega > 0 ? p->omega : 1.0 / ((3.0 * h->nu) + (1.0 / 2.0));
+    if (p->omega > 0) h->nu = (1.0 / p->omega - 0.5) / 3.0;
+    // bulk decomposition: waves of 62 written columns, strips of rows
+    h->ncg = (h->nx + 61) / 62;
+    h->ncb = (h->ncg + 3) / 4;
+    int H = p->strip_rows;
+    if (H <= 0) {
+        // aim for >= 8 bulk waves per SIMD slot of the 256 CUs, strips of 8..64 rows
+        const long long target = 256LL * 4 * 8;
+        H = 64;
+        while (H > 8 && (long long)h->ncg * ((h->ny + H - 1) / H) < target) H /= 2;
+    }
+    h->strip_rows = H;
+    h->nstrips = (h->ny + H - 1) / H;
+    if (hipStreamCreate(&h->st) != hipSuccess) return fail(MRLBM_E_HIP, "hipStreamCreate failed");
+    if (hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess)
+        return fail(MRLBM_E_HIP, "hipEventCreate failed");
+    const int rc = h->f32 ? create_typed<float, float>(h) : create_typed<double, double>(h);
+    if (rc) return fail(rc, "");
+    *out = h;
+    return MRLBM_OK;
+}
+
+extern "C" int mrlbm_destroy(mrlbm_handle *h)
+{
+    if (!h) return MRLBM_E_ARG;
+    (void)hipStreamSynchronize(h->st);
+    free_all(h);
+    delete h;
+    return MRLBM_OK;
+}
+
+extern "C" int mrlbm_get_info(const mrlbm_handle *h, mrlbm_info *o)
+{
+    if (!h || !o) return MRLBM_E_ARG;
+    std::memset(o, 0, sizeof *o);
+    o->nx = h->nx;
+    o->ny = h->ny;
+    o->nbct = h->bt.nbct;
+    o->precision = h->p.precision;
+    o->r_cyl = h->geo.r_cyl;
+    o->nu = h->nu;
+    o->omega = h->omega;
+    o->x_c = h->p.x_c;
+    o->y_c = h->p.y_c;
+    o->r_c = h->p.r_c;
+    o->bytes_per_step = (int64_t)12 * h->nx * h->ny * (h->f32 ? 4 : 8);
+    o->box[0] = h->bt.bx0;
+    o->box[1] = h->bt.by0;
+    o->box[2] = h->bt.bw;
+    o->box[3] = h->bt.bh;
+    o->n_stencil = h->bt.nU;
+    o->levels = h->bt.nlevels;
+    return MRLBM_OK;
+}
+
+extern "C" int mrlbm_get_boundary(const mrlbm_handle *h, int32_t *bi, int32_t *bj, int32_t *label,
+                                  double *theta, int32_t *idx)
+{
+    if (!h) return MRLBM_E_ARG;
+    const auto &bn = h->geo.bnodes;
+    for (size_t k = 0; k < bn.size(); k++) {
+        if (bi) bi[k] = bn[k].i + 1;
+        if (bj) bj[k] = bn[k].j + 1;
+        if (label) label[k] = bn[k].label;
+        if (theta) theta[k] = bn[k].theta;
+        if (idx) idx[k] = h->geo.idx[k] + 1;
+    }
+    return MRLBM_OK;
+}
+
+extern "C" int mrlbm_get_node_class(const mrlbm_handle *h, int8_t *cls)
+{
+    if (!h || !cls) return MRLBM_E_ARG;
+    if (h->geo.cls.empty()) std::memset(cls, 1, (size_t)h->nx * h->ny);
+    else std::memcpy(cls, h->geo.cls.data(), h->geo.cls.size());
+    return MRLBM_OK;
+}
+
+// dense (nx, ny) host <-> pitched device plane
+template <class S> static int put_plane(mrlbm_handle *h, S *dplane, const double *src, double shift)
+{
+    std::vector<S> tmp((size_t)h->pitch * h->ny, (S)0);
+    for (int j = 0; j < h->ny; j++)
+        for (int i = 0; i < h->nx; i++)
+            tmp[(size_t)j * h->pitch + i] = (S)(src[(size_t)j * h->nx + i] - shift);
+    HIPCHK(hipMemcpy(dplane, tmp.data(), tmp.size() * sizeof(S), hipMemcpyHostToDevice));
+    return 0;
+}
+template <class S> static int get_plane(const mrlbm_handle *h, const S *dplane, double *dst, double shift)
+{
+    std::vector<S> tmp((size_t)h->pitch * h->ny);
+    if (hipMemcpy(tmp.data(), dplane, tmp.size() * sizeof(S), hipMemcpyDeviceToHost) != hipSuccess)
+        return MRLBM_E_HIP;
+    for (int j = 0; j < h->ny; j++)
+        for (int i = 0; i < h->nx; i++)
+            dst[(size_t)j * h->nx + i] = shift + (double)tmp[(size_t)j * h->pitch + i];
+    return 0;
+}
+
+template <class S, class T>
+static int set_moments_typed(mrlbm_handle *h, const double *rho, const double *ux, const double *uy,
+                             const double *mxx, const double *myy, const double *mxy)
+{
+    const size_t N = (size_t)h->nx * h->ny;
+    std::vector<double> m[3];
+    for (int q = 0; q < 3; q++) m[q].resize(N);
+    for (size_t n = 0; n < N; n++) {
+        m[0][n] = mxx ? mxx[n] : ux[n] * ux[n];
+        m[1][n] = myy ? myy[n] : uy[n] * uy[n];
+        m[2][n] = mxy ? mxy[n] : ux[n] * uy[n];
+    }
+    S *d = (S *)h->S[h->cur];
+    int rc = put_plane<S>(h, d, rho, h->f32 ? 1.0 : 0.0);
+    rc |= put_plane<S>(h, d + h->plane, ux, 0.0);
+    rc |= put_plane<S>(h, d + 2 * h->plane, uy, 0.0);
+    for (int q = 0; q < 3; q++) rc |= put_plane<S>(h, d + (3 + q) * h->plane, m[q].data(), 0.0);
+    if (rc) return rc;
+    h->from_pops = false;
+    auto a = base_args<S, T>(h);
+    HIPCHK(launch_ghost_from_state<S, T>(a, (T *)h->ghost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+extern "C" int mrlbm_set_moments(mrlbm_handle *h, const double *rho, const double *ux, const double *uy,
+                                 const double *mxx, const double *myy, const double *mxy)
+{
+    if (!h || !rho || !ux || !uy) return MRLBM_E_ARG;
+    return h->f32 ? set_moments_typed<float, float>(h, rho, ux, uy, mxx, myy, mxy)
+                  : set_moments_typed<double, double>(h, rho, ux, uy, mxx, myy, mxy);
+}
+
+template <class S, class T> static int set_pops_typed(mrlbm_handle *h, const double *f)
+{
+    if (!h->pops) HIPCHK(hipMalloc(&h->pops, (size_t)Q * h->plane * sizeof(T)));
+    const size_t N = (size_t)h->nx * h->ny;
+    std::vector<T> tmp((size_t)h->pitch * h->ny, (T)0);
+    for (int k = 0; k < Q; k++) {
+        const double base = h->f32 ? wk<double>(k) : 0.0;
+        for (int j = 0; j < h->ny; j++)
+            for (int i = 0; i < h->nx; i++)
+                tmp[(size_t)j * h->pitch + i] = (T)(f[k * N + (size_t)j * h->nx + i] - base);
+        HIPCHK(hipMemcpy((T *)h->pops + k * h->plane, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+    h->from_pops = true;
+    auto a = base_args<S, T>(h);
+    HIPCHK(launch_ghost_from_pops<S, T>(a, (T *)h->ghost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+extern "C" int mrlbm_set_populations(mrlbm_handle *h, const double *f)
+{
+    if (!h || !f) return MRLBM_E_ARG;
+    return h->f32 ? set_pops_typed<float, float>(h, f) : set_pops_typed<double, double>(h, f);
+}
+
+static void drain_records(mrlbm_handle *h, const std::vector<int32_t> &iters, int nprobe, int record_p)
+{
+    const int n = (int)iters.size();
+    if (!n) return;
+    std::vector<double> buf((size_t)n * h->rec_stride);
+    if (hipMemcpyAsync(buf.data(), h->d_rec, buf.size() * sizeof(double), hipMemcpyDeviceToHost, h->st) != hipSuccess)
+        return;
+    (void)hipStreamSynchronize(h->st);
+    const int nb = h->bt.nbct;
+    for (int s = 0; s < n; s++) {
+        const double *r = &buf[(size_t)s * h->rec_stride];
+        if (h->has_b) {
+            h->q_force_iter.push_back(iters[s]);
+            h->q_fd.push_back(r[0]);
+            h->q_fl.push_back(r[1]);
+        }
+        if (nprobe > 0) {
+            h->q_probe_iter.push_back(iters[s]);
+            for (int q = 0; q < 12; q++) h->q_probe.push_back(r[2 + q]);
+        }
+        if (record_p && h->has_b) {
+            h->q_surf_iter.push_back(iters[s]);
+            h->q_ps.insert(h->q_ps.end(), r + 14, r + 14 + nb);
+            h->q_pfit.insert(h->q_pfit.end(), r + 14 + nb, r + 14 + 2 * nb);
+            h->last_p_iter = iters[s];
+            h->last_ps.assign(r + 14, r + 14 + nb);
+            h->last_pfit.assign(r + 14 + nb, r + 14 + 2 * nb);
+        }
+    }
+}
+
+template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_first, int nsteps, const mrlbm_step_opts *o)
+{
+    mrlbm_step_opts opts;
+    if (o) opts = *o;
+    else mrlbm_default_step_opts(&opts);
+    if (opts.nprobe < 0 || opts.nprobe > 4) FAIL(MRLBM_E_ARG, "nprobe must be 0..4");
+    int probe_node[4] = {-1, -1, -1, -1};
+    for (int p = 0; p < opts.nprobe; p++) {
+        const int i = opts.probe_i[p] - 1, j = opts.probe_j[p] - 1;
+        if (i < 0 || i >= h->nx || j < 0 || j >= h->ny) FAIL(MRLBM_E_ARG, "probe outside the grid");
+        probe_node[p] = i + h->pitch * j;
+    }
+    if (opts.post_process && opts.stats_interval > 0 && !h->d_acc) {
+        HIPCHK(hipMalloc(&h->d_acc, (size_t)5 * h->plane * sizeof(double)));
+        HIPCHK(hipMemset(h->d_acc, 0, (size_t)5 * h->plane * sizeof(double)));
+    }
+    if (opts.post_process) {
+        const int rc = ensure_rec(h);
+        if (rc) return rc;
+    }
+    const int march_blocks = h->ncb * h->nstrips;
+    const long long N = (long long)h->nx * h->ny;
+    const int simple_blocks = (int)((N + 255) / 256);
+    std::vector<int32_t> rec_iters;
+    if (h->timing) HIPCHK(hipEventRecord(h->e0, h->st));
+    for (int s = 0; s < nsteps; s++) {
+        const int iter = iter_first + s;
+        const bool window = opts.post_process && iter >= opts.statsbegin && iter <= opts.statsend;
+        auto a = base_args<S, T>(h);
+        a.zero_solid = (opts.post_process && opts.iplot > 0 && iter % opts.iplot == 0) ? 1 : 0;
+        a.sample = (window && opts.stats_interval > 0 && (iter - opts.statsbegin) % opts.stats_interval == 0) ? 1 : 0;
+        a.rec_slot = window ? (int)rec_iters.size() : -1;
+        a.record_p = window && opts.record_surface_p ? 1 : 0;
+        a.nprobe = opts.nprobe;
+        for (int p = 0; p < 4; p++) a.probe_node[p] = probe_node[p];
+        a.inlet_out = window ? h->d_inlet[iter & 1] : nullptr;
+        a.inlet_prev = (h->prev_inlet_valid && h->has_b) ? h->d_inlet[h->prev_inlet_buf] : nullptr;
+        a.inlet_prev_c = h->prev_inlet_c;
+        const bool pops_step = h->from_pops;
+        const int nb = ((pops_step || a.simple) ? simple_blocks : march_blocks) + (h->has_b ? 1 : 0);
+        HIPCHK(launch_step<S, T>(a, nb, h->st));
+        if (h->prev_inlet_valid && !h->has_b) {
+            auto f = a;
+            f.inlet_prev = h->d_inlet[h->prev_inlet_buf];
+            HIPCHK(launch_inlet_fold<S, T>(f, h->st));
+        }
+        h->prev_inlet_valid = window ? 1 : 0;
+        h->prev_inlet_buf = iter & 1;
+        h->prev_inlet_c = (double)(float)(iter - opts.statsbegin); // real() is REAL(4) (:358)
+        h->cur ^= 1;
+        h->from_pops = false;
+        if (a.sample) h->nsamples++;
+        if (window) {
+            rec_iters.push_back(iter);
+            if ((int)rec_iters.size() == h->rec_cap) {
+                drain_records(h, rec_iters, opts.nprobe, opts.record_surface_p);
+                rec_iters.clear();
+            }
+        }
+    }
+    if (h->prev_inlet_valid) { // fold the last window step's inlet density now
+        auto f = base_args<S, T>(h);
+        f.inlet_prev = h->d_inlet[h->prev_inlet_buf];
+        f.inlet_prev_c = h->prev_inlet_c;
+        HIPCHK(launch_inlet_fold<S, T>(f, h->st));
+        h->prev_inlet_valid = 0;
+    }
+    if (h->timing) {
+        HIPCHK(hipEventRecord(h->e1, h->st));
+        HIPCHK(hipEventSynchronize(h->e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, h->e0, h->e1));
+        h->last_ms = nsteps > 0 ? ms / nsteps : 0.0;
+    }
+    drain_records(h, rec_iters, opts.nprobe, opts.record_surface_p);
+    h->last_nprobe = opts.nprobe;
+    return 0;
+}
+
+extern "C" int mrlbm_step(mrlbm_handle *h, int32_t iter_first, int32_t nsteps, const mrlbm_step_opts *o)
+{
+    if (!h || nsteps < 0) return MRLBM_E_ARG;
+    return h->f32 ? step_typed<float, float>(h, iter_first, nsteps, o)
+                  : step_typed<double, double>(h, iter_first, nsteps, o);
+}
+
+extern "C" int mrlbm_synchronize(mrlbm_handle *h)
+{
+    if (!h) return MRLBM_E_ARG;
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+extern "C" int mrlbm_get_moments(const mrlbm_handle *hc, double *rho, double *ux, double *uy, double *mxx,
+                                 double *myy, double *mxy)
+{
+    auto *h = const_cast<mrlbm_handle *>(hc);
+    if (!h) return MRLBM_E_ARG;
+    if (h->from_pops) FAIL(MRLBM_E_STATE, "no step since mrlbm_set_populations");
+    HIPCHK(hipStreamSynchronize(h->st));
+    double *outs[6] = {rho, ux, uy, mxx, myy, mxy};
+    for (int q = 0; q < 6; q++) {
+        if (!outs[q]) continue;
+        const double shift = (q == 0 && h->f32) ? 1.0 : 0.0;
+        const int rc = h->f32 ? get_plane<float>(h, (const float *)h->S[h->cur] + q * h->plane, outs[q], shift)
+                              : get_plane<double>(h, (const double *)h->S[h->cur] + q * h->plane, outs[q], shift);
+        if (rc) FAIL(rc, "device to host copy failed");
+    }
+    return 0;
+}
+
+extern "C" int mrlbm_get_fields(const mrlbm_handle *h, double *rho, double *ux, double *uy)
+{
+    return mrlbm_get_moments(h, rho, ux, uy, nullptr, nullptr, nullptr);
+}
+
+template <class S, class T> static int get_pops_typed(mrlbm_handle *h, double *f)
+{
+    const size_t N = (size_t)h->nx * h->ny;
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, (size_t)Q * N * sizeof(double)));
+    auto a = base_args<S, T>(h);
+    hipError_t e = launch_get_populations<S, T>(a, d, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    if (e == hipSuccess) e = hipMemcpy(f, d, (size_t)Q * N * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    HIPCHK(e);
+    return 0;
+}
+
+extern "C" int mrlbm_get_populations(const mrlbm_handle *hc, double *f)
+{
+    auto *h = const_cast<mrlbm_handle *>(hc);
+    if (!h || !f) return MRLBM_E_ARG;
+    return h->f32 ? get_pops_typed<float, float>(h, f) : get_pops_typed<double, double>(h, f);
+}
+
+extern "C" int mrlbm_get_forces(mrlbm_handle *h, int32_t *iters, double *fd, double *fl, int32_t max,
+                                int32_t *count)
+{
+    if (!h || max < 0) return MRLBM_E_ARG;
+    const int n = std::min<int>(max, (int)h->q_fd.size());
+    for (int s = 0; s < n; s++) {
+        if (iters) iters[s] = h->q_force_iter[s];
+        if (fd) fd[s] = h->q_fd[s];
+        if (fl) fl[s] = h->q_fl[s];
+    }
+    h->q_force_iter.erase(h->q_force_iter.begin(), h->q_force_iter.begin() + n);
+    h->q_fd.erase(h->q_fd.begin(), h->q_fd.begin() + n);
+    h->q_fl.erase(h->q_fl.begin(), h->q_fl.begin() + n);
+    if (count) *count = n;
+    return 0;
+}
+
+extern "C" int mrlbm_get_surface_p(const mrlbm_handle *h, int32_t *iter, double *ps, double *pfit)
+{
+    if (!h) return MRLBM_E_ARG;
+    if (h->last_p_iter < 0) return MRLBM_E_STATE;
+    if (iter) *iter = h->last_p_iter;
+    if (ps) std::copy(h->last_ps.begin(), h->last_ps.end(), ps);
+    if (pfit) std::copy(h->last_pfit.begin(), h->last_pfit.end(), pfit);
+    return 0;
+}
+
+extern "C" int mrlbm_get_surface_p_series(mrlbm_handle *h, int32_t *iters, double *ps, double *pfit,
+                                          int32_t max, int32_t *count)
+{
+    if (!h || max < 0) return MRLBM_E_ARG;
+    const int nb = h->bt.nbct;
+    const int n = std::min<int>(max, (int)h->q_surf_iter.size());
+    for (int s = 0; s < n; s++) {
+        if (iters) iters[s] = h->q_surf_iter[s];
+        if (ps) std::copy(h->q_ps.begin() + (size_t)s * nb, h->q_ps.begin() + (size_t)(s + 1) * nb, ps + (size_t)s * nb);
+        if (pfit)
+            std::copy(h->q_pfit.begin() + (size_t)s * nb, h->q_pfit.begin() + (size_t)(s + 1) * nb, pfit + (size_t)s * nb);
+    }
+    h->q_surf_iter.erase(h->q_surf_iter.begin(), h->q_surf_iter.begin() + n);
+    h->q_ps.erase(h->q_ps.begin(), h->q_ps.begin() + (size_t)n * nb);
+    h->q_pfit.erase(h->q_pfit.begin(), h->q_pfit.begin() + (size_t)n * nb);
+    if (count) *count = n;
+    return 0;
+}
+
+extern "C" int mrlbm_get_probes(mrlbm_handle *h, int32_t *iters, double *vals, int32_t max, int32_t *count)
+{
+    if (!h || max < 0) return MRLBM_E_ARG;
+    const int n = std::min<int>(max, (int)h->q_probe_iter.size());
+    for (int s = 0; s < n; s++) {
+        if (iters) iters[s] = h->q_probe_iter[s];
+        if (vals) std::copy(h->q_probe.begin() + 12 * s, h->q_probe.begin() + 12 * (s + 1), vals + 12 * s);
+    }
+    h->q_probe_iter.erase(h->q_probe_iter.begin(), h->q_probe_iter.begin() + n);
+    h->q_probe.erase(h->q_probe.begin(), h->q_probe.begin() + 12 * (size_t)n);
+    if (count) *count = n;
+    return 0;
+}
+
+extern "C" int mrlbm_get_rho_inf_mean(const mrlbm_handle *hc, double *v)
+{
+    auto *h = const_cast<mrlbm_handle *>(hc);
+    if (!h || !v) return MRLBM_E_ARG;
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemcpy(v, h->d_rho_mean, sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int mrlbm_get_stats(const mrlbm_handle *hc, double *mux, double *muy, double *rux, double *ruy,
+                               int64_t *ns)
+{
+    auto *h = const_cast<mrlbm_handle *>(hc);
+    if (!h) return MRLBM_E_ARG;
+    if (ns) *ns = h->nsamples;
+    if (!h->d_acc || h->nsamples == 0) FAIL(MRLBM_E_STATE, "no statistics samples");
+    HIPCHK(hipStreamSynchronize(h->st));
+    const size_t N = (size_t)h->nx * h->ny;
+    std::vector<double> s[5];
+    for (int q = 0; q < 5; q++) {
+        s[q].resize(N);
+        const int rc = get_plane<double>(h, h->d_acc + q * h->plane, s[q].data(), 0.0);
+        if (rc) FAIL(rc, "device to host copy failed");
+    }
+    const double inv = 1.0 / (double)h->nsamples;
+    for (size_t n = 0; n < N; n++) {
+        const double mx = s[0][n] * inv, my = s[1][n] * inv;
+        if (mux) mux[n] = mx;
+        if (muy) muy[n] = my;
+        if (rux) rux[n] = std::sqrt(std::max(0.0, s[2][n] * inv - mx * mx));
+        if (ruy) ruy[n] = std::sqrt(std::max(0.0, s[3][n] * inv - my * my));
+    }
+    return 0;
+}
+
+extern "C" int mrlbm_reset_stats(mrlbm_handle *h)
+{
+    if (!h) return MRLBM_E_ARG;
+    h->nsamples = 0;
+    if (h->d_acc) HIPCHK(hipMemsetAsync(h->d_acc, 0, (size_t)5 * h->plane * sizeof(double), h->st));
+    return 0;
+}
+
+extern "C" int mrlbm_max_ux(const mrlbm_handle *hc, double *v)
+{
+    auto *h = const_cast<mrlbm_handle *>(hc);
+    if (!h || !v) return MRLBM_E_ARG;
+    std::vector<double> ux((size_t)h->nx * h->ny);
+    const int rc = mrlbm_get_moments(h, nullptr, ux.data(), nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    *v = *std::max_element(ux.begin(), ux.end());
+    return 0;
+}
+
+extern "C" int mrlbm_check_finite(const mrlbm_handle *hc)
+{
+    auto *h = const_cast<mrlbm_handle *>(hc);
+    if (!h) return MRLBM_E_ARG;
+    const size_t N = (size_t)h->nx * h->ny;
+    std::vector<double> m[6];
+    for (auto &v : m) v.resize(N);
+    const int rc = mrlbm_get_moments(h, m[0].data(), m[1].data(), m[2].data(), m[3].data(), m[4].data(), m[5].data());
+    if (rc) return rc;
+    for (auto &v : m)
+        for (double x : v)
+            if (!std::isfinite(x)) FAIL(MRLBM_E_NONFINITE, "non-finite moment in the state");
+    return 0;
+}
+
+extern "C" int mrlbm_set_timing(mrlbm_handle *h, int32_t on)
+{
+    if (!h) return MRLBM_E_ARG;
+    h->timing = on != 0;
+    return 0;
+}
+
+extern "C" int mrlbm_last_step_ms(const mrlbm_handle *h, double *ms)
+{
+    if (!h || !ms) return MRLBM_E_ARG;
+    *ms = h->last_ms;
+    return 0;
+}

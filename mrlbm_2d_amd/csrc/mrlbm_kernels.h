@@ -1,0 +1,67 @@
+// mrlbm_kernels.h -- kernel argument blocks and launch entry points (HIP, gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mrlbm {
+
+// Boundary workgroup tables and scratch (device pointers), see mrlbm_geometry.h.
+struct BoundaryDev {
+    int nbct, nU, nlevels;
+    const int32_t *node, *ij, *label, *uself, *uc, *dep, *level_order, *level_start;
+    const double *real;
+    const int32_t *unode, *uij;
+    const double *urot;
+    const int32_t *idx;
+    const double *thetas;
+    uint16_t incs[16], outs[16];
+    double wfx[16], wfy[16]; // rest-population force per label (FP32 form only)
+    int mass_bc, rotated, vel_interp, mom_interp;
+    void *U;      // 5*nU values of the step type: rho, ux, uy, Mxx', Myy' (pre-collision)
+    void *ub;     // 2*nbct interpolated velocities
+    double *pcy;  // nbct surface pressure, list order
+    double *fxy;  // 2*nbct per-node forces
+};
+
+template <class S, class T> struct StepArgs {
+    int nx, ny, pitch;
+    long long plane;
+    int xper, yper;
+    const S *src;
+    S *dst;
+    const T *pops;             // non-null: this step reads streamed populations [k*plane + node]
+    const T *ghost;            // stale populations: left[ny*9], right[ny*9], bottom[nx*9], top[nx*9]
+    T om, om1, vin;
+    const T *uin;              // inlet ux per row (ny)
+    int incomp_bc, incomp_bulk;
+    int bx0, by0, bw, bh;
+    const uint8_t *boxcls;     // 1 boundary node (skipped here), 2 solid node
+    int zero_solid;
+    int strip_rows, ncg, ncb, nstrips, has_boundary, simple;
+    // recording
+    int sample;                // accumulate mean/rms u
+    double *acc;               // 5 planes: ux, uy, ux^2, uy^2, ux*uy
+    int rec_slot;              // >= 0: record forces/probes/surface p into rec + slot*rec_stride
+    double *rec;
+    int rec_stride, record_p;
+    int nprobe, probe_node[4];
+    double *inlet_out;         // post-BC rho at i = 1 (ny), or null
+    const double *inlet_prev;  // previous window step's inlet rho, or null
+    double inlet_prev_c;       // its mean counter
+    double *rho_inf_mean;
+    BoundaryDev b;
+};
+
+// Host launchers (mrlbm.hip).
+template <class S, class T>
+hipError_t launch_step(const StepArgs<S, T> &a, int nblocks, hipStream_t st);
+template <class S, class T>
+hipError_t launch_inlet_fold(const StepArgs<S, T> &a, hipStream_t st);
+template <class S, class T>
+hipError_t launch_get_populations(const StepArgs<S, T> &a, double *f_dense, hipStream_t st);
+template <class S, class T>
+hipError_t launch_ghost_from_state(const StepArgs<S, T> &a, T *ghost, hipStream_t st);
+template <class S, class T>
+hipError_t launch_ghost_from_pops(const StepArgs<S, T> &a, T *ghost, hipStream_t st);
+
+} // namespace mrlbm
