@@ -20,6 +20,31 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
          "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas"]
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(OUT_DIR, "mrlbm_gpu")
+FC = os.environ.get("FC_MRLBM", "flang")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """Fortran drop-in driver (iso_c_binding) linked against libmrlbm.so."""
+    bdir = os.path.join(FORTRAN_DIR, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    iface_o = os.path.join(bdir, "mrlbm_iface.o")
+    steps = [
+        [FC, "-O2", "-J", bdir, "-c", os.path.join(FORTRAN_DIR, "mrlbm_iface.f90"), "-o", iface_o],
+        [FC, "-O2", "-J", bdir, os.path.join(FORTRAN_DIR, "mrlbm_driver.f90"), iface_o,
+         "-L", OUT_DIR, "-lmrlbm", "-Wl,-rpath,$ORIGIN", "-o", DRIVER + ".tmp"],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed:\n{r.stdout}{r.stderr}")
+        if verbose and (r.stdout + r.stderr).strip():
+            print(r.stdout + r.stderr, file=sys.stderr)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     tmp = LIB + ".tmp"
@@ -35,5 +60,9 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_all(verbose: bool = False):
+    return build(verbose), build_driver(verbose)
+
+
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build_all(verbose=True))

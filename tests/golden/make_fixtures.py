@@ -196,6 +196,63 @@ def run_case(name, overrides, nsteps, seed):
         print(f"{name}: {nx}x{ny} steps={nsteps} nbct={out.get('nbct')} ok", flush=True)
 
 
+STATS_BIN = os.path.join(REPO, "oracle", "_ref", "statistics_ref")
+
+
+def write_unformatted(path, iters, rows):
+    with open(path, "wb") as fh:
+        for it, row in zip(iters, rows):
+            body = np.array([it], np.int32).tobytes() + np.asarray(row, np.float64).tobytes()
+            m = np.array([len(body)], np.int32).tobytes()
+            fh.write(m + body + m)
+
+
+def run_statistics_case(name="stats_synth", nstep=1500, nbct=24, period=137, seed=5):
+    """The reference post-processor (2d_statistics.f90) on a synthetic shedding signal:
+    lift = A sin(2 pi t/period + phase) with a slow drift, drag = C0 + B cos(4 pi t/period),
+    surface pressure = smooth function of theta plus a periodic ripple."""
+    rng = np.random.default_rng(seed)
+    t = np.arange(nstep, dtype=np.float64)
+    ph = rng.uniform(0, 2 * np.pi)
+    fl = 0.02 * np.sin(2 * np.pi * t / period + ph) + 1e-6 * t / nstep
+    fd = 0.11 + 0.004 * np.cos(4 * np.pi * t / period + ph)
+    theta = np.sort(rng.uniform(0, 2 * np.pi, nbct))
+    ps = (1.0 / 3.0 + 0.01 * np.cos(theta)[None, :]
+          + 0.001 * np.sin(2 * np.pi * t / period)[:, None] * np.sin(theta)[None, :])
+    pfit = ps + 1e-4 * np.cos(3 * theta)[None, :]
+    uo, rho_infty, r_cy = 0.05, 1.0123, 20.39610
+    iters = np.arange(2, 2 + nstep, dtype=np.int32)
+    with tempfile.TemporaryDirectory() as d:
+        os.makedirs(os.path.join(d, "data_mean"))
+        os.makedirs(os.path.join(d, "plots"))
+        with open(os.path.join(d, "input_stat.dat"), "w") as fh:
+            fh.write(f"&parameters\n\tnbct = {nbct}\n\tr_cy = {r_cy!r}\n\tuo = {uo!r}\n"
+                     f"\trho_infty = {rho_infty!r}\n/\n")
+        with open(os.path.join(d, "data_mean", "forces_time.dat"), "w") as fh:
+            for it, a, b in zip(iters, fd, fl):
+                fh.write(f" {int(it)} {float(a)!r} {float(b)!r}\n")
+        with open(os.path.join(d, "data_mean", "theta.dat"), "w") as fh:
+            for x in theta:
+                fh.write(f" {float(x)!r}\n")
+        write_unformatted(os.path.join(d, "data_mean", "p_surface_time.dat"), iters, ps)
+        write_unformatted(os.path.join(d, "data_mean", "p_surface_smooth_time.dat"), iters, pfit)
+        r = subprocess.run([STATS_BIN], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(r.stdout + r.stderr)
+        vals = {}
+        for line in open(os.path.join(d, "plots", "drag_lift.dat")):
+            key, _, v = line.rpartition(":")
+            vals[key.strip()] = float(v)
+        cp = np.loadtxt(os.path.join(d, "plots", "cp.dat"))
+        cpf = np.loadtxt(os.path.join(d, "plots", "cp_fit.dat"))
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), fd=fd, fl=fl, theta=theta, ps=ps, pfit=pfit,
+                            uo=uo, rho_infty=rho_infty, r_cy=r_cy, iters=iters,
+                            st=vals["Strouhal number (St)"], cd=vals["Drag coefficinet (C_D)"],
+                            cl_rms=vals["Lift coefficinet (C_L"], cp=cp, cp_fit=cpf,
+                            params_json=np.array(json.dumps({"kind": "statistics"})))
+        print(f"{name}: St={vals['Strouhal number (St)']} Cd={vals['Drag coefficinet (C_D)']} ok", flush=True)
+
+
 def main(names=None):
     if not os.path.exists(REF_BIN):
         subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle")], check=True)
@@ -203,6 +260,8 @@ def main(names=None):
         if names and name not in names:
             continue
         run_case(name, ov, n, seed)
+    if not names or "stats_synth" in names:
+        run_statistics_case()
 
 
 if __name__ == "__main__":

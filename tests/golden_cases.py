@@ -10,8 +10,15 @@ import numpy as np
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def names():
-    return sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+def names(kind="run"):
+    """Fixture names of one kind: "run" (reference time-loop runs) or "statistics"."""
+    out = []
+    for p in sorted(glob.glob(os.path.join(GOLDEN, "*.npz"))):
+        z = np.load(p, allow_pickle=False)
+        k = json.loads(str(z["params_json"])).get("kind", "run")
+        if k == kind:
+            out.append(os.path.splitext(os.path.basename(p))[0])
+    return out
 
 
 def load(name):

@@ -1,0 +1,61 @@
+"""The Fortran drop-in driver (mrlbm_2d_amd/_lib/mrlbm_gpu) on the reference's own
+input.dat format, compared file by file with what the reference wrote for the same
+input (tests/golden): restart.dat record, forces_time.dat, surface-pressure records."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from mrlbm_2d_amd import build as B
+from tests import golden_cases as gc
+from tests.golden import make_fixtures as mf
+
+pytestmark = pytest.mark.gpu
+
+
+def run_driver(tmp_path, name):
+    g = gc.load(name)
+    p = dict(mf.BASE)
+    p.update(g["params"])
+    n = int(g["nsteps"])
+    d = str(tmp_path)
+    for sub in ("output", "restart", "data_probe", "data_geo", "data_mean"):
+        os.makedirs(os.path.join(d, sub), exist_ok=True)
+    irestart = 0
+    if "f0" in g:
+        mf.write_restart(os.path.join(d, "restart", "restart.dat"), 1, g["f0"], np.zeros((g["nx"], g["ny"])),
+                         np.zeros((g["nx"], g["ny"])), np.ones((g["nx"], g["ny"])))
+        irestart = 1
+    mf.write_input(os.path.join(d, "input.dat"), p, 1 + n, irestart)
+    print(f"[gpu] driver {name} steps={n}", flush=True)
+    r = subprocess.run([B.DRIVER], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return g, d, r.stdout
+
+
+@pytest.mark.parametrize("name", ["p1_walls", "s_noise_re60", "chan"])
+def test_driver_files_match_reference(tmp_path, name):
+    g, d, log = run_driver(tmp_path, name)
+    nx, ny = int(g["nx"]), int(g["ny"])
+    it, f, ux, uy, rho = mf.read_restart(os.path.join(d, "restart", "restart.dat"), nx, ny)
+    assert it == int(g["iter_last"])
+    np.testing.assert_array_equal(f, g["f"])
+    np.testing.assert_array_equal(rho, g["rho"])
+    np.testing.assert_array_equal(ux, g["ux"])
+    np.testing.assert_array_equal(uy, g["uy"])
+    assert "STEP:" in log
+    if "forces" in g:
+        fr = np.atleast_2d(np.loadtxt(os.path.join(d, "data_mean", "forces_time.dat")))
+        np.testing.assert_array_equal(fr[:, 0], g["forces"][:, 0])
+        scale = np.abs(g["forces"][:, 1:]).max()
+        np.testing.assert_allclose(fr[:, 1:], g["forces"][:, 1:], rtol=1e-12, atol=1e-12 * scale)
+        ps = mf.read_unformatted(os.path.join(d, "data_mean", "p_surface_time.dat"))
+        pf = mf.read_unformatted(os.path.join(d, "data_mean", "p_surface_smooth_time.dat"))
+        assert len(ps) == len(g["forces"])
+        np.testing.assert_allclose(ps[-1][1], g["ps_last"], rtol=1e-12)
+        np.testing.assert_allclose(pf[-1][1], g["pfit_last"], rtol=1e-12)
+        theta = np.loadtxt(os.path.join(d, "data_mean", "theta.dat"))
+        np.testing.assert_allclose(theta, g["theta"], rtol=0, atol=1e-15)
+        rm = np.atleast_2d(np.loadtxt(os.path.join(d, "rho_mean_inlet.dat")))
+        np.testing.assert_allclose(rm, g["rho_mean_inlet"], rtol=1e-14)
