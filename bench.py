@@ -8,18 +8,17 @@ and plain FP64.  A "step" is one lattice update of every node (one launch).
 Prints ONE JSON line.  `roofline` prices the step kernel against MI355X's 8 TB/s HBM:
 achieved = algorithmic bytes per launch (12 moment words per node: 6 read + 6 written)
 / mean kernel time measured with HIP events on the library's stream.
-`cpu_baseline` times the reference program itself (oracle/_ref/mrlbm_ref, built from
-the reference sources with flang, OpenMP) on a bounded number of steps of the same grid.
+`cpu_baseline` times the oracle (the CPU restatement of the reference step) on a bounded
+number of steps of the same grid.  The reference program itself is not timed here: its
+set-up writes every fluid node to formatted text files, which at this grid size runs for
+many minutes before the first step.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
-import shutil
-import subprocess
 import sys
-import tempfile
 import time
 
 import numpy as np
@@ -28,6 +27,10 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per bulk launch of the default variants at 8192 x 2048, from rocprofv3 FETCH_SIZE
+# and WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1);
+# the summary and the commands are in profiles/bulk_kernel_8192x2048_hbm_bytes.md
+TRAFFIC_BYTES = {"f32": 906_824_379, "f64": 1_923_698_699}
 NX, NY = 8192, 2048
 XC, YC, RC = 1024.0, 1025.0, 64.0
 UMAX, RE = 0.05, 200.0
@@ -74,98 +77,61 @@ def time_form(precision, steps, warmup, **kw):
                 bytes_per_launch=bytes_per_launch, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS)
 
 
-def reference_input(nsteps, threads):
-    # The reference's geometry for the same 8192 x 2048 grid: ncy = 128 (D = 128),
-    # L_front + 1 + L_back = 64, L_top + 1 + L_bot = 16 (2d_mrlbm.f90:86-92).
-    return f"""&DomainSize
- nx = 320
- ny = 32
- lattice_type = 9
-/
-&Cylinder
- ncy = 128
- L_front = 8
- L_back = 55
- L_top = 8
- L_bot = 7
-/
-&Numbers
- Re = {RE}
-/
-&Parallel
- num_threads={threads}
-/
-&Controls
- uo = {UMAX}
- rho_infty = 1.0
- iplot = 100000000
- max_iter = {1 + nsteps}
- isave = 100000000
- irestart = 0
- statsbegin = 100000000
- statsend = 100000001
- iplotstats = 1000
-/
-&LogicalControls
- post_process = .false.
- x_periodic = .false.
- y_periodic = .false.
- channel_with_cylinder = .true.
- channel_with_square = .false.
- incomp_bc = .false.
- incomp_bulk = .false.
- vel_interp = .true.
- mom_interp = .true.
- rotated_coordinate = .true.
- strong_mass_bc = .true.
- eq_mass_bc = .false.
- weak_mass_bc = .false.
-/
-"""
-
-
-def cpu_baseline(threads, n1=1, n2=6):
-    """Wall time of the reference program per step: two runs of n1 and n2 steps on the
-    8192 x 2048 grid, differenced so that setup and file output cancel."""
-    exe = os.path.join(REPO, "oracle", "_ref", "mrlbm_ref")
-    if not os.path.exists(exe):
-        return None
+def cpu_baseline(nsteps=2):
+    """Wall time per step of the oracle (the CPU restatement of the reference step, one
+    core) on the same 8192 x 2048 grid: one untimed step, then `nsteps` timed ones.  The
+    reference's own geometry (ncy = 128, L_front/back/top/bot = 8/55/8/7, 2d_mrlbm.f90:86-92)
+    and its uniform inlet; the per-step work does not depend on the inlet profile."""
+    from oracle import pyoracle as po
+    p = po.OracleParams(nx=NX, ny=NY, ncy=128, L_front=8, L_back=55, L_top=8, L_bot=7, Re=RE, uo=UMAX,
+                        post_process=0, iplot=10 ** 8, statsbegin=10 ** 8, statsend=10 ** 8 + 1)
+    o = po.Oracle(p)
+    assert (o.nx, o.ny) == (NX, NY)
+    o.run(2, 1)
+    print("[bench] CPU baseline: untimed step done", file=sys.stderr, flush=True)
     times = []
-    for n in (n1, n2):
-        d = tempfile.mkdtemp(prefix="mrlbm_ref_")
-        try:
-            for sub in ("output", "restart", "data_probe", "data_geo", "data_mean"):
-                os.makedirs(os.path.join(d, sub))
-            with open(os.path.join(d, "input.dat"), "w") as fh:
-                fh.write(reference_input(n, threads))
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
-            t0 = time.perf_counter()
-            r = subprocess.run([exe], cwd=d, env=env, capture_output=True, text=True, timeout=900)
-            times.append(time.perf_counter() - t0)
-            if r.returncode != 0:
-                return None
-        finally:
-            shutil.rmtree(d, ignore_errors=True)
-    per_step = (times[1] - times[0]) / (n2 - n1)
-    return dict(value=NX * NY / (per_step * 1e6), unit="MLUPS", cores=threads, kind="reference",
-                sample=f"reference program (flang -O3 -fopenmp, {threads} threads) on the same "
-                       f"8192x2048 grid (ncy=128), {n2} minus {n1} steps, wall clock differenced "
-                       f"({times[1]:.1f}s - {times[0]:.1f}s)")
+    for n in range(nsteps):
+        t0 = time.perf_counter()
+        o.run(3 + n, 1)
+        times.append(time.perf_counter() - t0)
+        print(f"[bench] CPU baseline: step {n + 1}/{nsteps} {times[-1]:.2f}s", file=sys.stderr, flush=True)
+    per_step = sum(times) / nsteps
+    del o
+    return dict(value=round(NX * NY / (per_step * 1e6), 3), unit="MLUPS", cores=1, kind="port",
+                sample=f"oracle (C restatement of the reference step, gcc -O2, 1 thread) on the same "
+                       f"8192x2048 grid, {nsteps} timed steps after one untimed step, "
+                       f"{per_step:.2f} s/step")
+
+
+def copy_calibration(nbytes, reps):
+    """A plain device copy of `nbytes` (torch's copy kernel), `reps` times: the known-size
+    transfer that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of a profiled run."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--strip-rows", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (for profiler counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU slab decomposition is "
-              "design-only (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: the step runs on one GPU; a multi-GPU slab decomposition along x is "
+              "designed but not built. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, max(args.steps, 1))
+        return
     from mrlbm_2d_amd import capi
     print(f"[bench] FP32 deviation form {NX}x{NY}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
@@ -176,9 +142,10 @@ def main():
     print(f"[bench] FP64: {r64}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
-        print("[bench] CPU baseline: reference program", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.cpu_threads)
+        print("[bench] CPU baseline: oracle", file=sys.stderr, flush=True)
+        cpu = cpu_baseline()
         print(f"[bench] CPU: {cpu}", file=sys.stderr, flush=True)
+    default_shape = args.strip_rows == 0
     out = {
         "metric": "MLUPS of the D2Q9 regularized LBM step, FP32 deviation form (FP64 beside it)",
         "value": round(r32["mlups"], 1),
@@ -192,13 +159,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded +-1e-4 velocity noise)",
-        "config": {"workload": "cfg3: 8192x2048 channel, cylinder D=128 at (1024,1025), "
+        "config": {"workload": "8192x2048 channel, cylinder D=128 at (1024,1025), "
                                "parabolic inlet u_max=0.05, Re=200", "nx": NX, "ny": NY},
         "roofline": {"bound": "hbm", "achieved": round(r32["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(r32["frac"], 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(r32["frac"], 4),
+                     "traffic": TRAFFIC_BYTES["f32"] if default_shape else None},
         "fp64": {"value": round(r64["mlups"], 1), "ms_per_step": round(r64["ms_per_step"], 5),
                  "roofline": {"bound": "hbm", "achieved": round(r64["achieved_gbs"], 1),
-                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(r64["frac"], 4)}},
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(r64["frac"], 4),
+                              "traffic": TRAFFIC_BYTES["f64"] if default_shape else None}},
         "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3),
         "cpu_baseline": cpu,
     }

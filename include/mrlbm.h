@@ -56,8 +56,12 @@ typedef struct mrlbm_params {
     int32_t label_policy;         /* MRLBM_LABEL_REFERENCE reproduces the reference (NaN on
                                      labels the reference leaves unset, :2008) */
     int32_t strip_rows;           /* tuning: rows per bulk strip (0 = auto) */
-    int32_t kernel;               /* 0: row-marching bulk kernel; 1: one node per thread (checking) */
-    int32_t reserved[6];
+    int32_t kernel;               /* bulk kernel: 0 auto (FP32: 304, FP64: 8), 1 one node per thread
+                                     (checking), 2 row loop over strip_rows rows, 4/8/16 unrolled strip
+                                     of that many rows with 62 written columns per wave, 304/308/316
+                                     aligned 64-column waves with an unrolled strip of kernel-300 rows */
+    int32_t block_order;          /* bulk blocks: 0 plain, 1 XCD-aware (vertical neighbours share an L2) */
+    int32_t reserved[5];
 } mrlbm_params;
 
 /* What one call to mrlbm_step records (2d_mrlbm.f90:355-502). */

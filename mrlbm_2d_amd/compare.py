@@ -1,0 +1,118 @@
+"""FP32 deviation form against FP64 on one cylinder configuration.
+
+Runs the same case in both precision forms (same seeded initial state), records the
+force series every step of the averaging window, and reports mean Cd, rms Cl and St
+(2d_statistics algorithm where two lift peaks exist, plain window means otherwise),
+the relative L2 difference of the final velocity field and both forms' throughput.
+
+    python -m mrlbm_2d_amd.compare --nx 8192 --ny 2048 --steps 100000 --window 50000
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import capi
+from . import statistics as S
+
+
+def make_params(nx, ny, xc, yc, rc, re, umax, precision, parabolic=True):
+    p = capi.default_params()
+    p.nx, p.ny = nx, ny
+    p.x_periodic, p.y_periodic = 0, 0
+    p.channel_with_cylinder = 1
+    p.x_c, p.y_c, p.r_c = xc, yc, rc
+    p.Re, p.uo = re, umax
+    p.inlet_profile = capi.INLET_PARABOLIC if parabolic else capi.INLET_UNIFORM
+    p.precision = precision
+    return p
+
+
+def run(params, steps, window, seed, chunk=2000):
+    lat = capi.Lattice(params)
+    nx, ny = lat.nx, lat.ny
+    rng = np.random.default_rng(seed)
+    ux = rng.uniform(-1e-4, 1e-4, (nx, ny))
+    uy = rng.uniform(-1e-4, 1e-4, (nx, ny))
+    lat.set_moments(np.ones((nx, ny)), ux, uy)
+    o = capi.default_step_opts()
+    o.post_process = 1
+    o.statsbegin = 2 + steps - window
+    o.statsend = 1 + steps
+    o.iplot = 0
+    fd, fl = [], []
+    t0 = time.perf_counter()
+    it = 2
+    while it <= 1 + steps:
+        n = min(chunk, 2 + steps - it)
+        lat.step(it, n, o)
+        _, a, b = lat.forces()
+        fd.append(a)
+        fl.append(b)
+        it += n
+    lat.synchronize()
+    wall = time.perf_counter() - t0
+    rho, ux, uy = lat.fields()
+    finite = lat.check_finite() == 0
+    info = lat.info
+    lat.close()
+    return dict(fd=np.concatenate(fd), fl=np.concatenate(fl), ux=ux, uy=uy, rho=rho, finite=finite,
+                mlups=nx * ny * steps / (wall * 1e6), r_cyl=info.r_cyl)
+
+
+def summarize(r, umax, rho_inf=1.0, parabolic=True):
+    uref = (2.0 / 3.0) * umax if parabolic else umax
+    out = dict(cd_mean=float(np.mean(r["fd"]) / (0.5 * rho_inf * uref * uref * 2 * r["r_cyl"])),
+               cl_rms=float(np.sqrt(np.mean(r["fl"] ** 2)) / (0.5 * rho_inf * uref * uref * 2 * r["r_cyl"])))
+    try:
+        s = S.compute(r["fd"], r["fl"], uref, rho_inf, r["r_cyl"])
+        out.update(st=s.st, cd_cycles=s.cd, cl_rms_cycles=s.cl_rms, period=s.period, n_cycle=s.n_cycle)
+    except S.StatisticsError:
+        out.update(st=None)
+    return out
+
+
+def compare(nx, ny, xc, yc, rc, re, umax, steps, window, seed=3, parabolic=True):
+    res = {}
+    runs = {}
+    for name, prec in (("fp64", capi.MRLBM_FP64), ("fp32", capi.MRLBM_FP32_DEV)):
+        r = run(make_params(nx, ny, xc, yc, rc, re, umax, prec, parabolic), steps, window, seed)
+        runs[name] = r
+        res[name] = dict(summarize(r, umax, parabolic=parabolic), mlups=r["mlups"], finite=r["finite"])
+    a, b = runs["fp64"], runs["fp32"]
+    du = np.sqrt(np.sum((a["ux"] - b["ux"]) ** 2 + (a["uy"] - b["uy"]) ** 2))
+    nu = np.sqrt(np.sum(a["ux"] ** 2 + a["uy"] ** 2))
+    res["l2_u_rel"] = float(du / nu)
+    res["cd_rel_err"] = abs(res["fp32"]["cd_mean"] - res["fp64"]["cd_mean"]) / abs(res["fp64"]["cd_mean"])
+    res["cl_rms_rel_err"] = (abs(res["fp32"]["cl_rms"] - res["fp64"]["cl_rms"]) / abs(res["fp64"]["cl_rms"])
+                             if res["fp64"]["cl_rms"] else None)
+    if res["fp64"].get("st") and res["fp32"].get("st"):
+        res["st_rel_err"] = abs(res["fp32"]["st"] - res["fp64"]["st"]) / res["fp64"]["st"]
+    res["fd_series_max_rel"] = float(np.max(np.abs(a["fd"] - b["fd"])) / np.max(np.abs(a["fd"])))
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nx", type=int, default=8192)
+    ap.add_argument("--ny", type=int, default=2048)
+    ap.add_argument("--xc", type=float, default=None)
+    ap.add_argument("--yc", type=float, default=None)
+    ap.add_argument("--rc", type=float, default=None)
+    ap.add_argument("--re", type=float, default=200.0)
+    ap.add_argument("--umax", type=float, default=0.05)
+    ap.add_argument("--steps", type=int, default=100000)
+    ap.add_argument("--window", type=int, default=50000)
+    ap.add_argument("--seed", type=int, default=3)
+    a = ap.parse_args()
+    xc = a.xc if a.xc is not None else a.nx / 8.0
+    yc = a.yc if a.yc is not None else a.ny / 2.0 + 1.0
+    rc = a.rc if a.rc is not None else a.ny / 32.0
+    print(json.dumps(compare(a.nx, a.ny, xc, yc, rc, a.re, a.umax, a.steps, a.window, a.seed)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

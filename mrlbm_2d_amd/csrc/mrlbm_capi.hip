@@ -35,7 +35,9 @@ struct mrlbm_handle {
     BoundaryTables bt;
     bool has_b = false;
     double nu = 0, omega = 0;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;   // bulk kernel and every host-facing operation
+    hipStream_t st2 = nullptr;  // curved-wall kernels, concurrent with the bulk
+    hipEvent_t evA = nullptr, evB = nullptr; // end of the last bulk / wall kernels
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool timing = false;
     double last_ms = 0;
@@ -46,6 +48,7 @@ struct mrlbm_handle {
     void *ghost = nullptr;
     void *uin = nullptr;
     uint8_t *d_box = nullptr;
+    unsigned long long *d_boxbits = nullptr;
     std::vector<void *> allocs; // boundary tables and scratch
     BoundaryDev bd{};
     double *d_rec = nullptr;
@@ -58,6 +61,7 @@ struct mrlbm_handle {
     int prev_inlet_valid = 0, prev_inlet_buf = 0;
     double prev_inlet_c = 0;
     int strip_rows = 32, ncg = 0, ncb = 0, nstrips = 0;
+    int variant = 0; // bulk kernel variant (see StepArgs::variant)
     // host-side queues of recorded series
     std::vector<int32_t> q_force_iter;
     std::vector<double> q_fd, q_fl;
@@ -142,12 +146,15 @@ template <class S, class T> static StepArgs<S, T> base_args(const mrlbm_handle *
     a.bw = h->bt.bw;
     a.bh = h->bt.bh;
     a.boxcls = h->d_box;
+    a.boxbits = h->d_boxbits;
     a.strip_rows = h->strip_rows;
     a.ncg = h->ncg;
     a.ncb = h->ncb;
     a.nstrips = h->nstrips;
     a.has_boundary = h->has_b ? 1 : 0;
     a.simple = h->p.kernel == 1 ? 1 : 0;
+    a.variant = h->variant;
+    a.xcd_order = (h->p.block_order == 1) ? 1 : 0;
     a.rec_slot = -1;
     a.rec = h->d_rec;
     a.rec_stride = h->rec_stride;
@@ -285,7 +292,7 @@ template <class S, class T> static int create_typed(mrlbm_handle *h)
 
 static void free_all(mrlbm_handle *h)
 {
-    for (void *p : {h->S[0], h->S[1], h->pops, h->ghost, h->uin, (void *)h->d_rec, (void *)h->d_inlet[0],
+    for (void *p : {h->S[0], h->S[1], h->pops, h->ghost, h->uin, (void *)h->d_boxbits, (void *)h->d_rec, (void *)h->d_inlet[0],
                     (void *)h->d_inlet[1], (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red})
         if (p) (void)hipFree(p);
     for (void *p : h->allocs)
@@ -293,7 +300,10 @@ static void free_all(mrlbm_handle *h)
     h->allocs.clear();
     if (h->e0) (void)hipEventDestroy(h->e0);
     if (h->e1) (void)hipEventDestroy(h->e1);
+    if (h->evA) (void)hipEventDestroy(h->evA);
+    if (h->evB) (void)hipEventDestroy(h->evB);
     if (h->st) (void)hipStreamDestroy(h->st);
+    if (h->st2) (void)hipStreamDestroy(h->st2);
 }
 
 extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
@@ -335,23 +345,59 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
     else h->nu = uref * (double)(p->ny - 1) / p->Re;
     h->omega = p->omega > 0 ? p->omega : 1.0 / ((3.0 * h->nu) + (1.0 / 2.0));
     if (p->omega > 0) h->nu = (1.0 / p->omega - 0.5) / 3.0;
-    // bulk decomposition: waves of 62 written columns, strips of rows
-    h->ncg = (h->nx + 61) / 62;
+    // bulk variant: 0 auto picks the fastest measured at 8192 x 2048 on MI355X
+    int kernel = p->kernel;
+    if (kernel == 0) kernel = h->f32 ? 304 : 8;
+    const bool aligned = kernel == 304 || kernel == 308 || kernel == 316;
+    // bulk decomposition: waves of 64 (aligned) or 62 written columns, strips of rows
+    h->ncg = aligned ? (h->nx + 63) / 64 : (h->nx + 61) / 62;
     h->ncb = (h->ncg + 3) / 4;
     int H = p->strip_rows;
     if (H <= 0) {
         // aim for >= 8 bulk waves per SIMD slot of the 256 CUs, strips of 8..64 rows
         const long long target = 256LL * 4 * 8;
         H = 64;
-        while (H > 8 && (long long)h->ncg * ((h->ny + H - 1) / H) < target) H /= 2;
+        while (H > 8 && (long long)h->ncg * ((h->ny - 2 + H - 1) / H) < target) H /= 2;
+    }
+    if (kernel == 4 || kernel == 8 || kernel == 16) {
+        h->variant = kernel; // unrolled strip of that many rows
+        H = kernel;
+    } else if (aligned) {
+        h->variant = kernel; // aligned 64-column waves, unrolled strip of kernel-300 rows
+        H = kernel - 300;
+    } else if (kernel == 1 || kernel == 2) {
+        h->variant = 0;      // 1: one node per thread (a.simple), 2: row loop over H rows
+    } else {
+        return fail(MRLBM_E_ARG, "kernel must be 0, 1, 2, 4, 8, 16, 304, 308 or 316");
     }
     h->strip_rows = H;
-    h->nstrips = (h->ny + H - 1) / H;
-    if (hipStreamCreate(&h->st) != hipSuccess) return fail(MRLBM_E_HIP, "hipStreamCreate failed");
-    if (hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess)
+    h->nstrips = (h->ny - 2 + H - 1) / H; // interior rows 1..ny-2; the edge kernel does the ring
+    if (hipStreamCreate(&h->st) != hipSuccess || hipStreamCreate(&h->st2) != hipSuccess)
+        return fail(MRLBM_E_HIP, "hipStreamCreate failed");
+    if (hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess ||
+        hipEventCreateWithFlags(&h->evA, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->evB, hipEventDisableTiming) != hipSuccess)
         return fail(MRLBM_E_HIP, "hipEventCreate failed");
+    if (hipEventRecord(h->evA, h->st) != hipSuccess || hipEventRecord(h->evB, h->st2) != hipSuccess)
+        return fail(MRLBM_E_HIP, "hipEventRecord failed");
     const int rc = h->f32 ? create_typed<float, float>(h) : create_typed<double, double>(h);
     if (rc) return fail(rc, "");
+    if (h->bt.bw > 0) { // per box column and bulk strip: 64-bit row masks (strips are <= 64 rows)
+        if (h->strip_rows > 64) return fail(MRLBM_E_ARG, "strip_rows must be <= 64");
+        std::vector<unsigned long long> bits((size_t)h->bt.bw * h->nstrips * 2, 0ull);
+        for (int bi = 0; bi < h->bt.bw; bi++)
+            for (int bj = 0; bj < h->bt.bh; bj++) {
+                const int j = h->bt.by0 + bj;
+                if (j < 1 || j > h->ny - 2) continue;
+                const int st = (j - 1) / h->strip_rows, t = (j - 1) % h->strip_rows;
+                const uint8_t c = h->bt.boxcls[(size_t)bj * h->bt.bw + bi];
+                if (c & 1) bits[((size_t)bi * h->nstrips + st) * 2] |= 1ull << t;
+                if (c & 2) bits[((size_t)bi * h->nstrips + st) * 2 + 1] |= 1ull << t;
+            }
+        if (hipMalloc(&h->d_boxbits, bits.size() * sizeof(unsigned long long)) != hipSuccess ||
+            hipMemcpy(h->d_boxbits, bits.data(), bits.size() * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(MRLBM_E_HIP, "box mask upload failed");
+    }
     *out = h;
     return MRLBM_OK;
 }
@@ -360,6 +406,7 @@ extern "C" int mrlbm_destroy(mrlbm_handle *h)
 {
     if (!h) return MRLBM_E_ARG;
     (void)hipStreamSynchronize(h->st);
+    (void)hipStreamSynchronize(h->st2);
     free_all(h);
     delete h;
     return MRLBM_OK;
@@ -496,6 +543,7 @@ static void drain_records(mrlbm_handle *h, const std::vector<int32_t> &iters, in
     const int n = (int)iters.size();
     if (!n) return;
     std::vector<double> buf((size_t)n * h->rec_stride);
+    if (hipStreamWaitEvent(h->st, h->evB, 0) != hipSuccess) return; // forces come from the wall stream
     if (hipMemcpyAsync(buf.data(), h->d_rec, buf.size() * sizeof(double), hipMemcpyDeviceToHost, h->st) != hipSuccess)
         return;
     (void)hipStreamSynchronize(h->st);
@@ -561,13 +609,20 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         a.inlet_prev = (h->prev_inlet_valid && h->has_b) ? h->d_inlet[h->prev_inlet_buf] : nullptr;
         a.inlet_prev_c = h->prev_inlet_c;
         const bool pops_step = h->from_pops;
-        const int nb = ((pops_step || a.simple) ? simple_blocks : march_blocks) + (h->has_b ? 1 : 0);
-        HIPCHK(launch_step<S, T>(a, nb, h->st));
-        if (h->prev_inlet_valid && !h->has_b) {
+        const int nb = (pops_step || a.simple) ? simple_blocks : march_blocks;
+        // bulk_n after wall_{n-1}; wall_n after bulk_{n-1}: both read state n-1 only
+        HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
+        HIPCHK(launch_bulk<S, T>(a, nb, h->st));
+        HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
+        HIPCHK(launch_edges<S, T>(a, h->st2));
+        if (h->has_b) HIPCHK(launch_boundary<S, T>(a, h->st2));
+        else if (h->prev_inlet_valid) {
             auto f = a;
             f.inlet_prev = h->d_inlet[h->prev_inlet_buf];
-            HIPCHK(launch_inlet_fold<S, T>(f, h->st));
+            HIPCHK(launch_inlet_fold<S, T>(f, h->st2));
         }
+        HIPCHK(hipEventRecord(h->evB, h->st2));
+        HIPCHK(hipEventRecord(h->evA, h->st));
         h->prev_inlet_valid = window ? 1 : 0;
         h->prev_inlet_buf = iter & 1;
         h->prev_inlet_c = (double)(float)(iter - opts.statsbegin); // real() is REAL(4) (:358)
@@ -582,6 +637,7 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             }
         }
     }
+    HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0)); // join the wall stream
     if (h->prev_inlet_valid) { // fold the last window step's inlet density now
         auto f = base_args<S, T>(h);
         f.inlet_prev = h->d_inlet[h->prev_inlet_buf];

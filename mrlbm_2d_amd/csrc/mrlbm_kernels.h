@@ -36,8 +36,12 @@ template <class S, class T> struct StepArgs {
     int incomp_bc, incomp_bulk;
     int bx0, by0, bw, bh;
     const uint8_t *boxcls;     // 1 boundary node (skipped here), 2 solid node
+    const unsigned long long *boxbits; // per box column and bulk strip: row bits of boundary, solid nodes
     int zero_solid;
     int strip_rows, ncg, ncb, nstrips, has_boundary, simple;
+    int variant;               // bulk kernel: 0 row loop (strip_rows), 4/8/16 unrolled strip of that height,
+                               // 304/308/316 aligned 64-column waves, unrolled strip of variant-300 rows
+    int xcd_order;             // 1: remap blocks so that each XCD takes consecutive strips of a column
     // recording
     int sample;                // accumulate mean/rms u
     double *acc;               // 5 planes: ux, uy, ux^2, uy^2, ux*uy
@@ -54,7 +58,11 @@ template <class S, class T> struct StepArgs {
 
 // Host launchers (mrlbm.hip).
 template <class S, class T>
-hipError_t launch_step(const StepArgs<S, T> &a, int nblocks, hipStream_t st);
+hipError_t launch_bulk(const StepArgs<S, T> &a, int nblocks, hipStream_t st);
+template <class S, class T>
+hipError_t launch_boundary(const StepArgs<S, T> &a, hipStream_t st);
+template <class S, class T>
+hipError_t launch_edges(const StepArgs<S, T> &a, hipStream_t st);
 template <class S, class T>
 hipError_t launch_inlet_fold(const StepArgs<S, T> &a, hipStream_t st);
 template <class S, class T>

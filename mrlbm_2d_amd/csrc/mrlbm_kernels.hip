@@ -1,10 +1,11 @@
 // mrlbm_kernels.hip -- gfx950 kernels of the D2Q9 moments-based regularized LBM step.
 //
-// One launch advances one time step.  Block 0 is the boundary workgroup: it recomputes
-// the pre-collision moments it needs from the previous state (stencil nodes of the
-// curved wall), applies the wall BC, collides and writes the boundary nodes, and reduces
-// the momentum-exchange forces and the surface-pressure feed.  All other blocks run the
-// bulk: each wave marches up a strip of rows over 64 columns (62 written, one halo
+// One time step = the bulk kernel on the main stream and, concurrently on a side stream,
+// the curved-wall kernels: stencil_moments_kernel recomputes the pre-collision moments
+// the wall needs from the previous state, boundary_kernel applies the wall BC, collides
+// and writes the boundary nodes, and reduces the momentum-exchange forces and the
+// surface-pressure feed.  Both read only the previous state and write disjoint nodes.
+// In the bulk each wave marches up a strip of rows over 64 columns (62 written, one halo
 // column per side), reconstructs the 9 outgoing populations of its own node from the 6
 // stored moments, exchanges the x-shifted ones with cross-lane shuffles, keeps the
 // y-shifted ones in registers from row to row, and writes the 6 new moments.  HBM
@@ -80,6 +81,27 @@ template <class S, class T> __device__ T pull_rule(const StepArgs<S, T> &a, int 
 template <class S, class T> __device__ void gather_generic(const StepArgs<S, T> &a, int i, int j, T f[Q])
 {
     for (int k = 0; k < Q; k++) f[k] = pull_rule(a, i, j, k);
+}
+
+// Gather at a node whose 8 neighbours are all inside the domain (the cylinder's stencil
+// and boundary nodes): all 54 loads are issued before any arithmetic, so one thread
+// waits for one memory round trip instead of nine.
+template <class S, class T> __device__ void gather_interior(const StepArgs<S, T> &a, int i, int j, T f[Q])
+{
+    if (a.pops) {
+#pragma unroll
+        for (int k = 0; k < Q; k++) f[k] = a.pops[(long long)k * a.plane + (long long)j * a.pitch + i];
+        return;
+    }
+    S m[Q][6];
+#pragma unroll
+    for (int k = 0; k < Q; k++) load_moments(a, i - EXk(k), j - EYk(k), m[k]);
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        T g[Q];
+        FormOf<T>::F::recon(m[k], g);
+        f[k] = g[k];
+    }
 }
 
 // Dirichlet values and the regularized edge BC, in the reference's order
@@ -165,79 +187,305 @@ __device__ __forceinline__ void node_update(const StepArgs<S, T> &a, int c, int 
     for (int p = 0; p < 6; p++) a.dst[p * a.plane + n] = (S)m[p];
 }
 
-template <class S, class T>
-__device__ __forceinline__ void load_row(const StepArgs<S, T> &a, int r, int cl, S m[6])
+// Workgroups are dealt to the 8 XCDs round-robin (block b -> XCD b % 8), each XCD with
+// its own L2.  With xcd_order the blocks an XCD receives are consecutive strips of one
+// column of waves, so the halo rows shared by vertically adjacent strips are read once
+// into that XCD's L2.  Returns (column block, strip).
+template <class S, class T> __device__ __forceinline__ int2 bulk_tile(const StepArgs<S, T> &a, int bb)
 {
-    if (r >= 0 && r < a.ny) load_moments(a, cl, r, m);
-    else rest_moments(m);
+    if (!a.xcd_order) return make_int2(bb % a.ncb, bb / a.ncb);
+    const int nb = a.ncb * a.nstrips;
+    const int x = bb & 7, k = bb >> 3;
+    const int L = x * (nb >> 3) + min(x, nb & 7) + k;
+    return make_int2(L / a.nstrips, L % a.nstrips);
+}
+
+// Row bits of the strip for this lane's column: boundary nodes (skipped by the bulk) and
+// solid nodes (u zeroed on plot steps).  Loaded once per strip, so the row loop touches
+// no mask memory.
+template <class S, class T>
+__device__ __forceinline__ void strip_masks(const StepArgs<S, T> &a, int c, int st, unsigned long long &bm,
+                                            unsigned long long &sm)
+{
+    bm = 0;
+    sm = 0;
+    const int bi = c - a.bx0;
+    if ((unsigned)bi < (unsigned)a.bw) {
+        const unsigned long long *p = a.boxbits + ((long long)bi * a.nstrips + st) * 2;
+        bm = p[0];
+        sm = p[1];
+    }
+}
+__device__ __forceinline__ int row_cls(unsigned long long bm, unsigned long long sm, int t)
+{
+    return (int)((bm >> t) & 1ull) | ((int)((sm >> t) & 1ull) << 1);
 }
 
 template <class T> __device__ __forceinline__ T shup(T v) { return __shfl_up(v, 1, 64); }
 template <class T> __device__ __forceinline__ T shdn(T v) { return __shfl_down(v, 1, 64); }
 
-// Bulk: one wave = 64 columns (62 written) marching up one strip of rows.
-template <class S, class T> __device__ void bulk_march(const StepArgs<S, T> &a, int bb)
+// One row of the bulk march for the wave's 64 columns: rebuild the populations of the
+// row above (moments in mN), assemble the 9 pulled populations of row j, finish the
+// interior nodes of row j, and roll the register window up by one row.
+template <class S, class T, bool REC>
+__device__ __forceinline__ void march_row(const StepArgs<S, T> &a, int c, bool valid, int j, const S mN[6],
+                                          T up_prev[3], T up_cur[3], T mid_cur[3], int cls)
+{
+    using F = typename FormOf<T>::F;
+    T F9[Q];
+    F::recon(mN, F9);
+    T f[Q];
+    f[0] = mid_cur[0];
+    f[1] = shup(mid_cur[1]);
+    f[3] = shdn(mid_cur[2]);
+    f[2] = up_prev[0];
+    f[5] = shup(up_prev[1]);
+    f[6] = shdn(up_prev[2]);
+    f[4] = F9[4];
+    f[7] = shdn(F9[7]);
+    f[8] = shup(F9[8]);
+    if (valid) {
+        if (!(cls & 1)) { // boundary nodes are written by the wall kernel
+            T m[6];
+            F::moments(f, m);
+            if (a.zero_solid && (cls & 2)) { // plot steps zero u on solid nodes (2d_mrlbm.f90:375-384)
+                m[1] = (T)0.0;
+                m[2] = (T)0.0;
+            }
+            const long long n = (long long)j * a.pitch + c;
+            if (REC) {
+                if (a.rec_slot >= 0)
+                    for (int p = 0; p < a.nprobe; p++)
+                        if (n == a.probe_node[p]) {
+                            double *r = a.rec + (long long)a.rec_slot * a.rec_stride + 2 + 3 * p;
+                            r[0] = F::dev ? 1.0 + (double)m[0] : (double)m[0];
+                            r[1] = m[1];
+                            r[2] = m[2];
+                        }
+                if (a.sample) {
+                    const double ux = m[1], uy = m[2];
+                    a.acc[n] += ux;
+                    a.acc[a.plane + n] += uy;
+                    a.acc[2 * a.plane + n] += ux * ux;
+                    a.acc[3 * a.plane + n] += uy * uy;
+                    a.acc[4 * a.plane + n] += ux * uy;
+                }
+            }
+            collide<T>(m, a.om, a.om1, a.incomp_bulk);
+#pragma unroll
+            for (int p = 0; p < 6; p++) a.dst[p * a.plane + n] = (S)m[p];
+        }
+    }
+    up_prev[0] = up_cur[0]; up_prev[1] = up_cur[1]; up_prev[2] = up_cur[2];
+    up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
+    mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
+}
+
+// Bulk (interior nodes 1..nx-2 x 1..ny-2): one wave = 64 columns (62 written, one halo
+// column per side) marching up a strip of rows.  The loop is unrolled by two with two
+// load buffers, so the loads of row j+2 are in flight while row j is computed and no
+// register copy ever waits on them.
+template <class S, class T, bool REC> __device__ void bulk_march(const StepArgs<S, T> &a, int bb)
 {
     using F = typename FormOf<T>::F;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int cbi = bb % a.ncb, st = bb / a.ncb;
+    const int2 tile = bulk_tile(a, bb);
+    const int cbi = tile.x, st = tile.y;
     const int g = cbi * 4 + wave;
     if (g >= a.ncg) return;
     const int c = g * 62 - 1 + lane;
     const int cl = min(max(c, 0), a.nx - 1);
-    const bool valid = lane >= 1 && lane <= 62 && c < a.nx;
-    const int j0 = st * a.strip_rows, j1 = min(a.ny, j0 + a.strip_rows);
-    S mN[6], mNN[6];
+    const bool valid = lane >= 1 && lane <= 62 && c >= 1 && c <= a.nx - 2;
+    const int j0 = 1 + st * a.strip_rows, j1 = min(a.ny - 1, j0 + a.strip_rows);
+    S mA[6], mB[6];
     T F9[Q], up_prev[3], up_cur[3], mid_cur[3];
-    load_row(a, j0 - 1, cl, mN);
-    F::recon(mN, F9);
+    load_moments(a, cl, j0 - 1, mA);
+    F::recon(mA, F9);
     up_prev[0] = F9[2]; up_prev[1] = F9[5]; up_prev[2] = F9[6];
-    load_row(a, j0, cl, mN);
-    F::recon(mN, F9);
+    load_moments(a, cl, j0, mA);
+    F::recon(mA, F9);
     up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
     mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
-    load_row(a, j0 + 1, cl, mN);
-    for (int j = j0; j < j1; j++) {
-        if (j + 2 <= j1) load_row(a, j + 2, cl, mNN);
-        F::recon(mN, F9);
-        T f[Q];
-        f[0] = mid_cur[0];
-        f[1] = shup(mid_cur[1]);
-        f[3] = shdn(mid_cur[2]);
-        f[2] = up_prev[0];
-        f[5] = shup(up_prev[1]);
-        f[6] = shdn(up_prev[2]);
-        f[4] = F9[4];
-        f[7] = shdn(F9[7]);
-        f[8] = shup(F9[8]);
-        if (valid) {
-            const bool edge = c == 0 || c == a.nx - 1 || j == 0 || j == a.ny - 1;
-            if (edge)
-                for (int k = 0; k < Q; k++) {
-                    const int si = c - EXk(k), sj = j - EYk(k);
-                    if (si < 0 || si >= a.nx || sj < 0 || sj >= a.ny) f[k] = pull_rule(a, c, j, k);
-                }
-            node_update(a, c, j, f, edge);
-        }
-        up_prev[0] = up_cur[0]; up_prev[1] = up_cur[1]; up_prev[2] = up_cur[2];
-        up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
-        mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
-#pragma unroll
-        for (int p = 0; p < 6; p++) mN[p] = mNN[p];
+    load_moments(a, cl, j0 + 1, mA);
+    unsigned long long bm, sm;
+    strip_masks(a, c, st, bm, sm);
+    for (int j = j0; j < j1; j += 2) {
+        if (j + 2 <= j1) load_moments(a, cl, j + 2, mB);
+        march_row<S, T, REC>(a, c, valid, j, mA, up_prev, up_cur, mid_cur, row_cls(bm, sm, j - j0));
+        if (j + 1 >= j1) break;
+        if (j + 3 <= j1) load_moments(a, cl, j + 3, mA);
+        march_row<S, T, REC>(a, c, valid, j + 1, mB, up_prev, up_cur, mid_cur, row_cls(bm, sm, j + 1 - j0));
     }
 }
 
+// Same march with the strip height a compile-time constant and the row loop fully
+// unrolled: every load of the strip is independent straight-line code, so the compiler
+// keeps many rows in flight without loop-carried copies of loaded registers.
+template <class S, class T, bool REC, int H> __device__ void bulk_march_u(const StepArgs<S, T> &a, int bb)
+{
+    using F = typename FormOf<T>::F;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int2 tile = bulk_tile(a, bb);
+    const int cbi = tile.x, st = tile.y;
+    const int g = cbi * 4 + wave;
+    if (g >= a.ncg) return;
+    const int c = g * 62 - 1 + lane;
+    const int cl = min(max(c, 0), a.nx - 1);
+    const bool valid = lane >= 1 && lane <= 62 && c >= 1 && c <= a.nx - 2;
+    const int j0 = 1 + st * H;
+    S m[H + 2][6];
+#pragma unroll
+    for (int t = 0; t < H + 2; t++) load_moments(a, cl, min(j0 - 1 + t, a.ny - 1), m[t]);
+    T F9[Q], up_prev[3], up_cur[3], mid_cur[3];
+    F::recon(m[0], F9);
+    up_prev[0] = F9[2]; up_prev[1] = F9[5]; up_prev[2] = F9[6];
+    F::recon(m[1], F9);
+    up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
+    mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
+    unsigned long long bm, sm;
+    strip_masks(a, c, st, bm, sm);
+#pragma unroll
+    for (int t = 0; t < H; t++)
+        march_row<S, T, REC>(a, c, valid && j0 + t <= a.ny - 2, j0 + t, m[t + 2], up_prev, up_cur, mid_cur,
+                             row_cls(bm, sm, t));
+}
+
+// Aligned march: a wave owns 64 aligned columns and writes all of them with full-wave
+// stores (a partially written 64-column span costs ~25% of HBM bandwidth here).  The two
+// halo columns (x0-1 and x0+64) are loaded by every lane at a per-lane address (the lane's
+// own column for lanes 1..62, so those loads coalesce with the main ones), rebuilt, and
+// selected in by lanes 0 and 63 in place of the cross-lane shuffle.
+template <class S, class T, bool REC>
+__device__ __forceinline__ void march_row_a(const StepArgs<S, T> &a, int c, bool valid, int j, const S mN[6],
+                                            const S hN[6], bool left, bool right, T up_prev[3], T up_cur[3],
+                                            T mid_cur[3], T hu_prev[1], T hu_cur[1], T hm_cur[1], int cls)
+{
+    using F = typename FormOf<T>::F;
+    T F9[Q], H9[Q];
+    F::recon(mN, F9);
+    F::recon(hN, H9);
+    // halo populations that move into the wave: left halo sends 1, 5, 8; right sends 3, 6, 7
+    const T hm_next = left ? H9[1] : H9[3];
+    const T hu_next = left ? H9[5] : H9[6];
+    const T hd = left ? H9[8] : H9[7];
+    T f[Q];
+    f[0] = mid_cur[0];
+    const T s1 = shup(mid_cur[1]), s3 = shdn(mid_cur[2]);
+    const T s5 = shup(up_prev[1]), s6 = shdn(up_prev[2]);
+    const T s7 = shdn(F9[7]), s8 = shup(F9[8]);
+    f[1] = left ? hm_cur[0] : s1;
+    f[3] = right ? hm_cur[0] : s3;
+    f[2] = up_prev[0];
+    f[5] = left ? hu_prev[0] : s5;
+    f[6] = right ? hu_prev[0] : s6;
+    f[4] = F9[4];
+    f[7] = right ? hd : s7;
+    f[8] = left ? hd : s8;
+    if (valid && !(cls & 1)) {
+        T m[6];
+        F::moments(f, m);
+        if (a.zero_solid && (cls & 2)) {
+            m[1] = (T)0.0;
+            m[2] = (T)0.0;
+        }
+        const long long n = (long long)j * a.pitch + c;
+        if (REC) {
+            if (a.rec_slot >= 0)
+                for (int p = 0; p < a.nprobe; p++)
+                    if (n == a.probe_node[p]) {
+                        double *r = a.rec + (long long)a.rec_slot * a.rec_stride + 2 + 3 * p;
+                        r[0] = F::dev ? 1.0 + (double)m[0] : (double)m[0];
+                        r[1] = m[1];
+                        r[2] = m[2];
+                    }
+            if (a.sample) {
+                const double ux = m[1], uy = m[2];
+                a.acc[n] += ux;
+                a.acc[a.plane + n] += uy;
+                a.acc[2 * a.plane + n] += ux * ux;
+                a.acc[3 * a.plane + n] += uy * uy;
+                a.acc[4 * a.plane + n] += ux * uy;
+            }
+        }
+        collide<T>(m, a.om, a.om1, a.incomp_bulk);
+#pragma unroll
+        for (int p = 0; p < 6; p++) a.dst[p * a.plane + n] = (S)m[p];
+    }
+    up_prev[0] = up_cur[0]; up_prev[1] = up_cur[1]; up_prev[2] = up_cur[2];
+    up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
+    mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
+    hu_prev[0] = hu_cur[0];
+    hu_cur[0] = hu_next;
+    hm_cur[0] = hm_next;
+}
+
+template <class S, class T, bool REC, int H> __device__ void bulk_march_a(const StepArgs<S, T> &a, int bb)
+{
+    using F = typename FormOf<T>::F;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int2 tile = bulk_tile(a, bb);
+    const int cbi = tile.x, st = tile.y;
+    const int g = cbi * 4 + wave;
+    if (g >= a.ncg) return;
+    const int x0 = g * 64, c = x0 + lane;          // c < pitch: the pitch is a multiple of 64
+    const bool left = lane == 0, right = lane == 63;
+    const int hc = left ? max(x0 - 1, 0) : (right ? min(x0 + 64, a.pitch - 1) : c);
+    const bool valid = c >= 1 && c <= a.nx - 2;
+    const int j0 = 1 + st * H;
+    S m[H + 2][6], h[H + 2][6];
+#pragma unroll
+    for (int t = 0; t < H + 2; t++) {
+        const int r = min(j0 - 1 + t, a.ny - 1);
+        load_moments(a, c, r, m[t]);
+        load_moments(a, hc, r, h[t]);
+    }
+    T F9[Q], H9[Q], up_prev[3], up_cur[3], mid_cur[3], hu_prev[1], hu_cur[1], hm_cur[1];
+    F::recon(m[0], F9);
+    F::recon(h[0], H9);
+    up_prev[0] = F9[2]; up_prev[1] = F9[5]; up_prev[2] = F9[6];
+    hu_prev[0] = left ? H9[5] : H9[6];
+    F::recon(m[1], F9);
+    F::recon(h[1], H9);
+    up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
+    mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
+    hu_cur[0] = left ? H9[5] : H9[6];
+    hm_cur[0] = left ? H9[1] : H9[3];
+    unsigned long long bm, sm;
+    strip_masks(a, c, st, bm, sm);
+#pragma unroll
+    for (int t = 0; t < H; t++)
+        march_row_a<S, T, REC>(a, c, valid && j0 + t <= a.ny - 2, j0 + t, m[t + 2], h[t + 2], left, right, up_prev,
+                               up_cur, mid_cur, hu_prev, hu_cur, hm_cur, row_cls(bm, sm, t));
+}
+
 // Bulk, one node per thread with the generic gather: the first step after
-// mrlbm_set_populations (populations in memory) and a debugging path.
+// mrlbm_set_populations (populations in memory) and a checking path.
 template <class S, class T> __device__ void bulk_simple(const StepArgs<S, T> &a, int bb)
 {
     const long long t = (long long)bb * blockDim.x + threadIdx.x;
     if (t >= (long long)a.nx * a.ny) return;
     const int c = (int)(t % a.nx), j = (int)(t / a.nx);
+    if (c == 0 || c == a.nx - 1 || j == 0 || j == a.ny - 1) return; // edge kernel
     T f[Q];
     gather_generic(a, c, j, f);
-    const bool edge = c == 0 || c == a.nx - 1 || j == 0 || j == a.ny - 1;
-    node_update(a, c, j, f, edge);
+    node_update(a, c, j, f, false);
+}
+
+// Edge nodes (the domain's outer ring): inlet, outlet, channel walls or periodic wraps,
+// stale populations, in the reference's BC order (2d_mrlbm.f90:247-351).
+template <class S, class T> __global__ void __launch_bounds__(256) edge_kernel(StepArgs<S, T> a)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = a.nx, ny = a.ny;
+    if (r >= 2 * nx + 2 * (ny - 2)) return;
+    int c, j;
+    if (r < nx) { c = r; j = 0; }
+    else if (r < 2 * nx) { c = r - nx; j = ny - 1; }
+    else if (r < 2 * nx + ny - 2) { c = 0; j = 1 + r - 2 * nx; }
+    else { c = nx - 1; j = 1 + r - 2 * nx - (ny - 2); }
+    T f[Q];
+    gather_generic(a, c, j, f);
+    node_update(a, c, j, f, true);
 }
 
 // ---------------------------------------------------------------------------------
@@ -396,7 +644,32 @@ __device__ double quadratic_fit(double r1, double p1, double r2, double p2, doub
     return (a0 + (a1 * rs)) + (a2 * (rs * rs));
 }
 
-template <class S, class T> __device__ void boundary_block(const StepArgs<S, T> &a)
+// Pre-collision moments at the stencil nodes of the curved wall (the reference's moments
+// sweep at those nodes, 2d_mrlbm.f90:231-245; same arithmetic as the bulk, so identical).
+template <class S, class T> __global__ void __launch_bounds__(256) stencil_moments_kernel(StepArgs<S, T> a)
+{
+    using F = typename FormOf<T>::F;
+    const BoundaryDev &b = a.b;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= b.nU) return;
+    T *U = (T *)b.U;
+    const int nU = b.nU;
+    T f[Q], m[6];
+    gather_interior(a, b.uij[2 * u], b.uij[2 * u + 1], f);
+    F::moments(f, m);
+    U[u] = m[0];
+    U[nU + u] = m[1];
+    U[2 * nU + u] = m[2];
+    if (b.rotated && b.mom_interp) { // rotated moments at the stencil node (:840-869)
+        const T c2 = (T)b.urot[3 * u], s2 = (T)b.urot[3 * u + 1], sc = (T)b.urot[3 * u + 2];
+        U[3 * nU + u] = ((m[3] * c2) + (m[4] * s2)) + (m[5] * sc);
+        U[4 * nU + u] = ((m[3] * s2) + (m[4] * c2)) - (m[5] * sc);
+    }
+}
+
+// Wall BC, collision, store, momentum exchange and surface-pressure feed of the boundary
+// nodes (one workgroup), after stencil_moments_kernel of the same step.
+template <class S, class T> __global__ void __launch_bounds__(256) boundary_kernel(StepArgs<S, T> a)
 {
     using F = typename FormOf<T>::F;
     const BoundaryDev &b = a.b;
@@ -404,21 +677,6 @@ template <class S, class T> __device__ void boundary_block(const StepArgs<S, T> 
     T *U = (T *)b.U, *ub = (T *)b.ub;
     const int tid = threadIdx.x, nt = blockDim.x, nU = b.nU;
     const T om = a.om;
-    // 1. pre-collision moments at the stencil nodes (the reference's moments sweep, :231-245)
-    for (int u = tid; u < nU; u += nt) {
-        T f[Q], m[6];
-        gather_generic(a, b.uij[2 * u], b.uij[2 * u + 1], f);
-        F::moments(f, m);
-        U[u] = m[0];
-        U[nU + u] = m[1];
-        U[2 * nU + u] = m[2];
-        if (b.rotated && b.mom_interp) { // rotated moments at the stencil node (:840-869)
-            const T c2 = (T)b.urot[3 * u], s2 = (T)b.urot[3 * u + 1], sc = (T)b.urot[3 * u + 2];
-            U[3 * nU + u] = ((m[3] * c2) + (m[4] * s2)) + (m[5] * sc);
-            U[4 * nU + u] = ((m[3] * s2) + (m[4] * c2)) - (m[5] * sc);
-        }
-    }
-    __syncthreads();
     // 2. velocity at the boundary nodes, level by level so that a node reads the
     //    already-interpolated velocity of an earlier node, as the in-place loop does (:773-815)
     const bool interp_u = b.vel_interp || !b.rotated;
@@ -456,7 +714,7 @@ template <class S, class T> __device__ void boundary_block(const StepArgs<S, T> 
         const int i = b.ij[2 * k], j = b.ij[2 * k + 1];
         const double *R = b.real + (long long)k * BT_NREAL;
         T fp[Q], ff[Q];
-        gather_generic(a, i, j, fp);
+        gather_interior(a, i, j, fp);
 #pragma unroll
         for (int q = 0; q < Q; q++) ff[q] = F::dev ? fp[q] + F::base(q) : fp[q];
         const T ux = ub[2 * k], uy = ub[2 * k + 1];
@@ -585,17 +843,18 @@ template <class S, class T> __device__ void boundary_block(const StepArgs<S, T> 
     }
 }
 
-template <class S, class T> __global__ void __launch_bounds__(256) step_kernel(StepArgs<S, T> a)
+// H: 0 row loop over a runtime strip height, 4/8/16 unrolled strip of H rows (62 written
+// columns per wave), 304/308/316 aligned 64-column waves with an unrolled strip of H-300 rows.
+template <class S, class T, bool REC, int H> __global__ void __launch_bounds__(256) bulk_kernel(StepArgs<S, T> a)
 {
-    if (a.has_boundary) {
-        if (blockIdx.x == 0) {
-            boundary_block(a);
-            return;
-        }
-    }
-    const int bb = blockIdx.x - a.has_boundary;
-    if (a.simple || a.pops) bulk_simple(a, bb);
-    else bulk_march(a, bb);
+    if constexpr (H >= 300) bulk_march_a<S, T, REC, H - 300>(a, blockIdx.x);
+    else if constexpr (H == 0) bulk_march<S, T, REC>(a, blockIdx.x);
+    else bulk_march_u<S, T, REC, H>(a, blockIdx.x);
+}
+
+template <class S, class T> __global__ void __launch_bounds__(256) bulk_simple_kernel(StepArgs<S, T> a)
+{
+    bulk_simple(a, blockIdx.x);
 }
 
 template <class S, class T> __global__ void __launch_bounds__(256) inlet_fold_kernel(StepArgs<S, T> a)
@@ -658,9 +917,34 @@ __global__ void __launch_bounds__(256) ghost_pops_kernel(StepArgs<S, T> a, T *gh
 }
 
 // ---------------------------------------------------------------------------------
-template <class S, class T> hipError_t launch_step(const StepArgs<S, T> &a, int nblocks, hipStream_t st)
+template <class S, class T> hipError_t launch_bulk(const StepArgs<S, T> &a, int nblocks, hipStream_t st)
 {
-    hipLaunchKernelGGL((step_kernel<S, T>), dim3(nblocks), dim3(256), 0, st, a);
+    const bool rec = a.sample || (a.rec_slot >= 0 && a.nprobe > 0);
+    const dim3 g(nblocks), b(256);
+#define MRLBM_BULK(R, H) hipLaunchKernelGGL((bulk_kernel<S, T, R, H>), g, b, 0, st, a)
+    if (a.simple || a.pops) hipLaunchKernelGGL((bulk_simple_kernel<S, T>), g, b, 0, st, a);
+    else if (rec && a.variant >= 300) MRLBM_BULK(true, 304);
+    else if (rec) MRLBM_BULK(true, 0);
+    else if (a.variant == 4) MRLBM_BULK(false, 4);
+    else if (a.variant == 8) MRLBM_BULK(false, 8);
+    else if (a.variant == 16) MRLBM_BULK(false, 16);
+    else if (a.variant == 304) MRLBM_BULK(false, 304);
+    else if (a.variant == 308) MRLBM_BULK(false, 308);
+    else if (a.variant == 316) MRLBM_BULK(false, 316);
+    else MRLBM_BULK(false, 0);
+#undef MRLBM_BULK
+    return hipGetLastError();
+}
+template <class S, class T> hipError_t launch_edges(const StepArgs<S, T> &a, hipStream_t st)
+{
+    const int n = 2 * a.nx + 2 * (a.ny - 2);
+    hipLaunchKernelGGL((edge_kernel<S, T>), dim3((n + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+template <class S, class T> hipError_t launch_boundary(const StepArgs<S, T> &a, hipStream_t st)
+{
+    hipLaunchKernelGGL((stencil_moments_kernel<S, T>), dim3((a.b.nU + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((boundary_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 template <class S, class T> hipError_t launch_inlet_fold(const StepArgs<S, T> &a, hipStream_t st)
@@ -688,7 +972,9 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
 }
 
 #define MRLBM_INSTANTIATE(S, T)                                                                  \
-    template hipError_t launch_step<S, T>(const StepArgs<S, T> &, int, hipStream_t);             \
+    template hipError_t launch_bulk<S, T>(const StepArgs<S, T> &, int, hipStream_t);             \
+    template hipError_t launch_boundary<S, T>(const StepArgs<S, T> &, hipStream_t);              \
+    template hipError_t launch_edges<S, T>(const StepArgs<S, T> &, hipStream_t);                 \
     template hipError_t launch_inlet_fold<S, T>(const StepArgs<S, T> &, hipStream_t);            \
     template hipError_t launch_get_populations<S, T>(const StepArgs<S, T> &, double *, hipStream_t); \
     template hipError_t launch_ghost_from_state<S, T>(const StepArgs<S, T> &, T *, hipStream_t);  \

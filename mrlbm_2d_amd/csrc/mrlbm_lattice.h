@@ -97,16 +97,16 @@ __device__ __forceinline__ void recon9_dev(float d, float ux, float uy, float mx
     const float c1 = 3.0f * ux, c2 = 3.0f * uy;
     const float c5 = 3.0f * (ux + uy), c6 = 3.0f * (uy - ux), c8 = 3.0f * (ux - uy);
     const float r = 1.0f + d;
-    const float B = XB + YB;
-    e[0] = Lat<float>::w0 * (d + r * (XA + YA));
-    e[1] = Lat<float>::w1 * (d + r * ((c1 + XB) + YA));
-    e[2] = Lat<float>::w1 * (d + r * ((c2 + XA) + YB));
-    e[3] = Lat<float>::w1 * (d + r * ((XB - c1) + YA));
-    e[4] = Lat<float>::w1 * (d + r * ((XA - c2) + YB));
-    e[5] = Lat<float>::w5 * (d + r * ((c5 + B) + Z));
-    e[6] = Lat<float>::w5 * (d + r * ((c6 + B) - Z));
-    e[7] = Lat<float>::w5 * (d + r * ((B - c5) + Z));
-    e[8] = Lat<float>::w5 * (d + r * ((c8 + B) - Z));
+    const float P = XB + YA, Qv = XA + YB, Bp = (XB + YB) + Z, Bm = (XB + YB) - Z;
+    e[0] = Lat<float>::w0 * __builtin_fmaf(r, XA + YA, d);
+    e[1] = Lat<float>::w1 * __builtin_fmaf(r, P + c1, d);
+    e[2] = Lat<float>::w1 * __builtin_fmaf(r, Qv + c2, d);
+    e[3] = Lat<float>::w1 * __builtin_fmaf(r, P - c1, d);
+    e[4] = Lat<float>::w1 * __builtin_fmaf(r, Qv - c2, d);
+    e[5] = Lat<float>::w5 * __builtin_fmaf(r, Bp + c5, d);
+    e[6] = Lat<float>::w5 * __builtin_fmaf(r, Bm + c6, d);
+    e[7] = Lat<float>::w5 * __builtin_fmaf(r, Bp - c5, d);
+    e[8] = Lat<float>::w5 * __builtin_fmaf(r, Bm + c8, d);
 }
 
 __device__ __forceinline__ void moments_dev(const float e[Q], float &d, float &ux, float &uy, float &mxx,
@@ -118,10 +118,10 @@ __device__ __forceinline__ void moments_dev(const float e[Q], float &d, float &u
     d = ((e[0] + (a13 + a24)) + diag) + DEV_MASS_F;
     const float sx = (e[1] - e[3]) + ((e[5] - e[7]) + (e[8] - e[6]));
     const float sy = (e[2] - e[4]) + ((e[5] - e[7]) + (e[6] - e[8]));
-    const float sxx = HA * (e[0] + a24) + HB * (a13 + diag);
-    const float syy = HA * (e[0] + a13) + HB * (a24 + diag);
+    const float sxx = __builtin_fmaf(HA, e[0] + a24, HB * (a13 + diag));
+    const float syy = __builtin_fmaf(HA, e[0] + a13, HB * (a24 + diag));
     const float sxy = a57 - a68;
-    const float inv = 1.0f / (1.0f + d);
+    const float inv = __builtin_amdgcn_rcpf(1.0f + d); // 1 ulp reciprocal
     ux = sx * inv;
     uy = sy * inv;
     mxx = sxx * inv;
@@ -156,6 +156,24 @@ struct FormF32 {
 
 // BGK relaxation of the Hermite moments (2d_mrlbm.f90:410-412), then the optional
 // incompressible trace projection whose result feeds the reconstruction (:419-428).
+template <class T>
+__device__ __forceinline__ void collide(T m[6], T om, T om1, int incomp_bulk);
+
+// FP32 form: the same relaxation with fused multiply-adds.
+template <>
+__device__ __forceinline__ void collide<float>(float m[6], float om, float om1, int incomp_bulk)
+{
+    const float ux = m[1], uy = m[2];
+    m[3] = __builtin_fmaf(om * ux, ux, om1 * m[3]);
+    m[4] = __builtin_fmaf(om * uy, uy, om1 * m[4]);
+    m[5] = __builtin_fmaf(om * ux, uy, om1 * m[5]);
+    if (incomp_bulk) {
+        const float tr = 0.5f * (((m[3] + m[4]) - ux * ux) - uy * uy);
+        m[3] = m[3] - tr;
+        m[4] = m[4] - tr;
+    }
+}
+
 template <class T>
 __device__ __forceinline__ void collide(T m[6], T om, T om1, int incomp_bulk)
 {

@@ -17,8 +17,8 @@ module mrlbm_iface
         integer(c_int32_t) :: inlet_profile, mass_bc, rotated_coordinate
         integer(c_int32_t) :: vel_interp, mom_interp
         integer(c_int32_t) :: incomp_bc, incomp_bulk
-        integer(c_int32_t) :: precision, label_policy, strip_rows, kernel
-        integer(c_int32_t) :: reserved(6)
+        integer(c_int32_t) :: precision, label_policy, strip_rows, kernel, block_order
+        integer(c_int32_t) :: reserved(5)
     end type
 
     type, bind(C) :: mrlbm_step_opts
