@@ -155,6 +155,44 @@ int mrlbm_max_ux(const mrlbm_handle *h, double *value);
 /* MRLBM_E_NONFINITE when any stored moment is NaN/Inf (the reference never checks). */
 int mrlbm_check_finite(const mrlbm_handle *h);
 
+/* Brinkman solid fraction chi(nx,ny), column-major (no counterpart in the reference): before
+ * the collision of every non-edge, non-wall node the velocity is scaled, u <- (1 - chi) u.
+ * The physical range is [0,1); any finite value below 1 is accepted (finite-difference
+ * checks step below 0).  NULL removes the field.  While a field is set the bulk runs the
+ * one-node-per-thread kernel. */
+int mrlbm_set_brinkman(mrlbm_handle *h, const double *chi);
+
+/* Discrete adjoint of the step for the time-averaged drag (no counterpart in the
+ * reference; objective built on evaluate_forces, 2d_mrlbm.f90:558-591, and the Cd
+ * normalisation of 2d_statistics.f90:29-32). */
+typedef struct mrlbm_adjoint_opts {
+    int32_t iter_first;           /* iteration number of the first forward step */
+    int32_t nsteps;               /* forward steps from the current state */
+    int32_t window;               /* J = mean Cd over the last `window` steps */
+    int32_t checkpoint_interval;  /* K: moment fields are checkpointed every K steps and the
+                                     K steps in between are recomputed in the reverse sweep */
+    int32_t reserved[4];
+} mrlbm_adjoint_opts;
+
+typedef struct mrlbm_adjoint_result {
+    double J;                     /* mean of Fx / (0.5 * 1 * uref^2 * 2 r_cyl) over the window,
+                                     uref = 2/3 uo (parabolic inlet) or uo (uniform) */
+    double dJ_duo;                /* dJ/d(uo) at fixed omega (inlet profile and Cd scale) */
+    double forward_ms;            /* checkpointed forward run, wall clock */
+    double adjoint_ms;            /* reverse sweep (recomputation + adjoint steps), wall clock */
+    double reserved[4];
+} mrlbm_adjoint_result;
+
+void mrlbm_default_adjoint_opts(mrlbm_adjoint_opts *o);
+/* Runs the forward steps from the current state (the handle ends in the final state),
+ * then the adjoint sweep back to the start.  dJ_dchi (nx*ny, column-major, may be NULL)
+ * receives dJ/dchi; nodes on the edges and the curved wall, where chi does not act, get 0.
+ * Requires a cylinder, walls (no periodic axis), incomp_bulk = 0, and a state set by
+ * mrlbm_set_moments or reached by stepping (not mrlbm_set_populations).  Uses a zero
+ * Brinkman field when none is set. */
+int mrlbm_adjoint_drag(mrlbm_handle *h, const mrlbm_adjoint_opts *opts, double *dJ_dchi,
+                       mrlbm_adjoint_result *result);
+
 /* Mean kernel time (ms) of the bulk step over the last mrlbm_step call, from HIP events
  * recorded on the handle's stream; 0 when timing is off. */
 int mrlbm_set_timing(mrlbm_handle *h, int32_t on);

@@ -28,6 +28,7 @@ EXPORTS = (
     "mrlbm_get_surface_p", "mrlbm_get_surface_p_series", "mrlbm_get_rho_inf_mean",
     "mrlbm_get_probes", "mrlbm_get_stats", "mrlbm_reset_stats", "mrlbm_max_ux",
     "mrlbm_check_finite", "mrlbm_set_timing", "mrlbm_last_step_ms",
+    "mrlbm_set_brinkman", "mrlbm_default_adjoint_opts", "mrlbm_adjoint_drag",
 )
 
 
@@ -62,6 +63,20 @@ class Info(C.Structure):
         ("x_c", C.c_double), ("y_c", C.c_double), ("r_c", C.c_double),
         ("bytes_per_step", C.c_int64), ("box", C.c_int32 * 4), ("n_stencil", C.c_int32),
         ("levels", C.c_int32),
+    ]
+
+
+class AdjointOpts(C.Structure):
+    _fields_ = [
+        ("iter_first", C.c_int32), ("nsteps", C.c_int32), ("window", C.c_int32),
+        ("checkpoint_interval", C.c_int32), ("reserved", C.c_int32 * 4),
+    ]
+
+
+class AdjointResult(C.Structure):
+    _fields_ = [
+        ("J", C.c_double), ("dJ_duo", C.c_double), ("forward_ms", C.c_double),
+        ("adjoint_ms", C.c_double), ("reserved", C.c_double * 4),
     ]
 
 
@@ -112,6 +127,10 @@ def load(path: str = LIB_PATH):
     L.mrlbm_check_finite.argtypes = [vp]
     L.mrlbm_set_timing.argtypes = [vp, C.c_int32]
     L.mrlbm_last_step_ms.argtypes = [vp, dp]
+    L.mrlbm_set_brinkman.argtypes = [vp, dp]
+    L.mrlbm_default_adjoint_opts.argtypes = [C.POINTER(AdjointOpts)]
+    L.mrlbm_default_adjoint_opts.restype = None
+    L.mrlbm_adjoint_drag.argtypes = [vp, C.POINTER(AdjointOpts), dp, C.POINTER(AdjointResult)]
     _lib = L
     return L
 
@@ -125,6 +144,12 @@ def default_params() -> Params:
 def default_step_opts() -> StepOpts:
     o = StepOpts()
     load().mrlbm_default_step_opts(C.byref(o))
+    return o
+
+
+def default_adjoint_opts() -> AdjointOpts:
+    o = AdjointOpts()
+    load().mrlbm_default_adjoint_opts(C.byref(o))
     return o
 
 
@@ -203,6 +228,21 @@ class Lattice:
         sh = (self.nx, self.ny)
         arrs = [_fort(x, sh) if x is not None else None for x in (rho, ux, uy, mxx, myy, mxy)]
         self._check(self._L.mrlbm_set_moments(self._h, *[_dp(x) for x in arrs]))
+
+    def set_brinkman(self, chi):
+        """Brinkman solid fraction (nx, ny); None removes it."""
+        arr = _fort(chi, (self.nx, self.ny)) if chi is not None else None
+        self._check(self._L.mrlbm_set_brinkman(self._h, _dp(arr)))
+
+    def adjoint_drag(self, iter_first: int, nsteps: int, window: int, checkpoint_interval: int):
+        """Forward run with checkpoints, then the adjoint sweep.  Returns (result, dJ/dchi)
+        with dJ/dchi as an (nx, ny) array."""
+        o = default_adjoint_opts()
+        o.iter_first, o.nsteps, o.window, o.checkpoint_interval = iter_first, nsteps, window, checkpoint_interval
+        g = np.zeros(self.N)
+        r = AdjointResult()
+        self._check(self._L.mrlbm_adjoint_drag(self._h, C.byref(o), _dp(g), C.byref(r)))
+        return r, g.reshape(self.nx, self.ny, order="F")
 
     def step(self, iter_first: int, nsteps: int, opts: StepOpts | None = None):
         self._check(self._L.mrlbm_step(self._h, int(iter_first), int(nsteps),

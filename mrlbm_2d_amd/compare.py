@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import sys
 import time
 
 import numpy as np
@@ -53,6 +54,8 @@ def run(params, steps, window, seed, chunk=2000):
         fd.append(a)
         fl.append(b)
         it += n
+        if (it - 2) % (10 * chunk) == 0:
+            print(f"[compare] precision {params.precision}: {it - 2} of {steps} steps", file=sys.stderr, flush=True)
     lat.synchronize()
     wall = time.perf_counter() - t0
     rho, ux, uy = lat.fields()

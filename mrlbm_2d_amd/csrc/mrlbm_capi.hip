@@ -1,7 +1,5 @@
 // mrlbm_capi.hip -- the C-ABI of include/mrlbm.h: handles, device memory, the step loop.
-#include "mrlbm.h"
-#include "mrlbm_geometry.h"
-#include "mrlbm_kernels.h"
+#include "mrlbm_handle.h"
 #include "mrlbm_lattice.h"
 
 #include <hip/hip_runtime.h>
@@ -12,78 +10,9 @@
 #include <string>
 #include <vector>
 
-using namespace mrlbm;
-
 namespace {
 thread_local std::string g_create_error;
-
-template <class T> T *dmalloc(size_t n, hipError_t &e)
-{
-    void *p = nullptr;
-    e = hipMalloc(&p, n ? n * sizeof(T) : sizeof(T));
-    return (T *)p;
-}
 } // namespace
-
-struct mrlbm_handle {
-    mrlbm_params p{};
-    std::string err;
-    int nx = 0, ny = 0, pitch = 0;
-    long long plane = 0;
-    bool f32 = false;
-    Geometry geo;
-    BoundaryTables bt;
-    bool has_b = false;
-    double nu = 0, omega = 0;
-    hipStream_t st = nullptr;   // bulk kernel and every host-facing operation
-    hipStream_t st2 = nullptr;  // curved-wall kernels, concurrent with the bulk
-    hipEvent_t evA = nullptr, evB = nullptr; // end of the last bulk / wall kernels
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    bool timing = false;
-    double last_ms = 0;
-    void *S[2] = {nullptr, nullptr};
-    int cur = 0;
-    void *pops = nullptr;
-    bool from_pops = false;
-    void *ghost = nullptr;
-    void *uin = nullptr;
-    uint8_t *d_box = nullptr;
-    unsigned long long *d_boxbits = nullptr;
-    std::vector<void *> allocs; // boundary tables and scratch
-    BoundaryDev bd{};
-    double *d_rec = nullptr;
-    int rec_cap = 0, rec_stride = 0;
-    double *d_inlet[2] = {nullptr, nullptr};
-    double *d_rho_mean = nullptr;
-    double *d_acc = nullptr;
-    int64_t nsamples = 0;
-    double *d_red = nullptr; // reduction scratch
-    int prev_inlet_valid = 0, prev_inlet_buf = 0;
-    double prev_inlet_c = 0;
-    int strip_rows = 32, ncg = 0, ncb = 0, nstrips = 0;
-    int variant = 0; // bulk kernel variant (see StepArgs::variant)
-    // host-side queues of recorded series
-    std::vector<int32_t> q_force_iter;
-    std::vector<double> q_fd, q_fl;
-    std::vector<int32_t> q_probe_iter;
-    std::vector<double> q_probe;
-    std::vector<int32_t> q_surf_iter;
-    std::vector<double> q_ps, q_pfit;
-    int last_p_iter = -1;
-    std::vector<double> last_ps, last_pfit;
-    int last_nprobe = 0;
-};
-
-#define FAIL(code, msg)            \
-    do {                           \
-        h->err = (msg);            \
-        return (code);             \
-    } while (0)
-#define HIPCHK(...)                                                                     \
-    do {                                                                                \
-        hipError_t e_ = (__VA_ARGS__);                                                  \
-        if (e_ != hipSuccess) FAIL(MRLBM_E_HIP, std::string(#__VA_ARGS__) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 extern "C" int mrlbm_abi_version(void) { return MRLBM_ABI_VERSION; }
 
@@ -120,48 +49,6 @@ extern "C" void mrlbm_default_step_opts(mrlbm_step_opts *o)
 extern "C" const char *mrlbm_last_error(const mrlbm_handle *h)
 {
     return h ? h->err.c_str() : g_create_error.c_str();
-}
-
-template <class S, class T> static StepArgs<S, T> base_args(const mrlbm_handle *h)
-{
-    StepArgs<S, T> a{};
-    a.nx = h->nx;
-    a.ny = h->ny;
-    a.pitch = h->pitch;
-    a.plane = h->plane;
-    a.xper = h->p.x_periodic;
-    a.yper = h->p.y_periodic;
-    a.src = (const S *)h->S[h->cur];
-    a.dst = (S *)h->S[h->cur ^ 1];
-    a.pops = h->from_pops ? (const T *)h->pops : nullptr;
-    a.ghost = (const T *)h->ghost;
-    a.om = (T)h->omega;
-    a.om1 = (T)(1.0 - h->omega);
-    a.vin = (T)0.0;
-    a.uin = (const T *)h->uin;
-    a.incomp_bc = h->p.incomp_bc;
-    a.incomp_bulk = h->p.incomp_bulk;
-    a.bx0 = h->bt.bx0;
-    a.by0 = h->bt.by0;
-    a.bw = h->bt.bw;
-    a.bh = h->bt.bh;
-    a.boxcls = h->d_box;
-    a.boxbits = h->d_boxbits;
-    a.strip_rows = h->strip_rows;
-    a.ncg = h->ncg;
-    a.ncb = h->ncb;
-    a.nstrips = h->nstrips;
-    a.has_boundary = h->has_b ? 1 : 0;
-    a.simple = h->p.kernel == 1 ? 1 : 0;
-    a.variant = h->variant;
-    a.xcd_order = (h->p.block_order == 1) ? 1 : 0;
-    a.rec_slot = -1;
-    a.rec = h->d_rec;
-    a.rec_stride = h->rec_stride;
-    a.acc = h->d_acc;
-    a.rho_inf_mean = h->d_rho_mean;
-    a.b = h->bd;
-    return a;
 }
 
 template <class T> static int upload(mrlbm_handle *h, void **dst, const std::vector<T> &v)
@@ -292,8 +179,9 @@ template <class S, class T> static int create_typed(mrlbm_handle *h)
 
 static void free_all(mrlbm_handle *h)
 {
+    adjoint_free(h);
     for (void *p : {h->S[0], h->S[1], h->pops, h->ghost, h->uin, (void *)h->d_boxbits, (void *)h->d_rec, (void *)h->d_inlet[0],
-                    (void *)h->d_inlet[1], (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red})
+                    (void *)h->d_inlet[1], (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red, h->chi})
         if (p) (void)hipFree(p);
     for (void *p : h->allocs)
         if (p) (void)hipFree(p);
@@ -532,6 +420,34 @@ template <class S, class T> static int set_pops_typed(mrlbm_handle *h, const dou
     return 0;
 }
 
+template <class T> static int set_chi_typed(mrlbm_handle *h, const double *chi)
+{
+    std::vector<T> tmp((size_t)h->plane, (T)0);
+    for (int j = 0; j < h->ny; j++)
+        for (int i = 0; i < h->nx; i++) {
+            const double v = chi[(size_t)j * h->nx + i];
+            if (!(v < 1.0) || !std::isfinite(v)) FAIL(MRLBM_E_ARG, "Brinkman fraction must be finite and below 1");
+            tmp[(size_t)j * h->pitch + i] = (T)v;
+        }
+    if (!h->chi) HIPCHK(hipMalloc(&h->chi, (size_t)h->plane * sizeof(T)));
+    HIPCHK(hipMemcpy(h->chi, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int mrlbm_set_brinkman(mrlbm_handle *h, const double *chi)
+{
+    if (!h) return MRLBM_E_ARG;
+    if (!chi) {
+        if (h->chi) {
+            HIPCHK(hipStreamSynchronize(h->st));
+            HIPCHK(hipFree(h->chi));
+        }
+        h->chi = nullptr;
+        return 0;
+    }
+    return h->f32 ? set_chi_typed<float>(h, chi) : set_chi_typed<double>(h, chi);
+}
+
 extern "C" int mrlbm_set_populations(mrlbm_handle *h, const double *f)
 {
     if (!h || !f) return MRLBM_E_ARG;
@@ -609,7 +525,7 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         a.inlet_prev = (h->prev_inlet_valid && h->has_b) ? h->d_inlet[h->prev_inlet_buf] : nullptr;
         a.inlet_prev_c = h->prev_inlet_c;
         const bool pops_step = h->from_pops;
-        const int nb = (pops_step || a.simple) ? simple_blocks : march_blocks;
+        const int nb = (pops_step || a.simple || a.chi) ? simple_blocks : march_blocks;
         // bulk_n after wall_{n-1}; wall_n after bulk_{n-1}: both read state n-1 only
         HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
         HIPCHK(launch_bulk<S, T>(a, nb, h->st));

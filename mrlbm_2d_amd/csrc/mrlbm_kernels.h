@@ -53,6 +53,7 @@ template <class S, class T> struct StepArgs {
     const double *inlet_prev;  // previous window step's inlet rho, or null
     double inlet_prev_c;       // its mean counter
     double *rho_inf_mean;
+    const T *chi;              // Brinkman solid fraction (per-node bulk kernel only), or null
     BoundaryDev b;
 };
 
@@ -71,5 +72,7 @@ template <class S, class T>
 hipError_t launch_ghost_from_state(const StepArgs<S, T> &a, T *ghost, hipStream_t st);
 template <class S, class T>
 hipError_t launch_ghost_from_pops(const StepArgs<S, T> &a, T *ghost, hipStream_t st);
+template <class S, class T>
+hipError_t launch_stencil_moments(const StepArgs<S, T> &a, hipStream_t st);
 
 } // namespace mrlbm

@@ -1,0 +1,171 @@
+"""Discrete adjoint of the time-averaged drag (mrlbm_adjoint_drag): gradients with respect
+to the Brinkman solid fraction field and the inlet velocity, checked against central
+finite differences of the forward run (FP64), against the FP64 adjoint (FP32 form), and
+through a torch.autograd.Function wrapper (the only place torch meets the adjoint)."""
+import numpy as np
+import pytest
+
+from mrlbm_2d_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+NX, NY = 96, 40
+XC, YC, RC = 24.0, 20.5, 4.5          # half-integer centre, as the 512x128 adjoint case
+RE, UO = 40.0, 0.05
+NSTEPS, WINDOW, K = 300, 150, 64        # 300 = 4*64 + 44: a partial last segment
+
+
+def params(precision=capi.MRLBM_FP64, uo=UO, omega=0.0):
+    p = capi.default_params()
+    p.nx, p.ny = NX, NY
+    p.x_periodic, p.y_periodic = 0, 0
+    p.channel_with_cylinder = 1
+    p.x_c, p.y_c, p.r_c = XC, YC, RC
+    p.Re, p.uo = RE, uo
+    p.omega = omega
+    p.inlet_profile = capi.INLET_PARABOLIC
+    p.precision = precision
+    return p
+
+
+def bumps(seed, amp, n=6):
+    """Smooth Gaussian bumps within 2D of the cylinder centre."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(1, NX + 1)[:, None]
+    y = np.arange(1, NY + 1)[None, :]
+    chi = np.zeros((NX, NY))
+    for _ in range(n):
+        r = rng.uniform(RC + 1.5, 4 * RC)
+        th = rng.uniform(0, 2 * np.pi)
+        cx, cy = XC + r * np.cos(th), YC + r * np.sin(th)
+        chi += amp * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * 2.0 ** 2))
+    return chi
+
+
+def drag_objective(p, chi):
+    """J = mean Cd over the window from a plain forward run."""
+    with capi.Lattice(p) as lat:
+        lat.set_brinkman(chi)
+        o = capi.default_step_opts()
+        o.post_process = 1
+        o.statsbegin = 2 + NSTEPS - WINDOW
+        o.statsend = 1 + NSTEPS
+        lat.step(2, NSTEPS, o)
+        _, fd, _ = lat.forces()
+        assert len(fd) == WINDOW
+        uref = (2.0 / 3.0) * p.uo
+        return float(np.sum(fd) / WINDOW / (0.5 * uref * uref * 2 * lat.info.r_cyl))
+
+
+def adjoint(p, chi):
+    with capi.Lattice(p) as lat:
+        lat.set_brinkman(chi)
+        res, g = lat.adjoint_drag(2, NSTEPS, WINDOW, K)
+        omega = lat.info.omega
+        return res, g, omega
+
+
+@pytest.fixture(scope="module")
+def fp64():
+    chi = bumps(7, 0.01)
+    res, g, omega = adjoint(params(), chi)
+    print(f"[gpu] adjoint fp64: J {res.J:.10f} dJ/duo {res.dJ_duo:.6e} fwd {res.forward_ms:.2f} ms "
+          f"adj {res.adjoint_ms:.2f} ms", flush=True)
+    return chi, res, g, omega
+
+
+def test_objective_matches_forward_run(fp64):
+    chi, res, _, _ = fp64
+    # same forces; only the order of the window sum differs
+    assert abs(res.J - drag_objective(params(), chi)) <= 1e-14 * abs(res.J)
+
+
+def test_gradient_zero_where_chi_has_no_effect(fp64):
+    _, _, g, _ = fp64
+    with capi.Lattice(params()) as lat:
+        cls = lat.node_class()
+    assert np.all(g[0, :] == 0) and np.all(g[-1, :] == 0) and np.all(g[:, 0] == 0) and np.all(g[:, -1] == 0)
+    assert np.all(g[cls == 3] == 0)
+    assert np.abs(g).max() > 0
+
+
+def test_directional_derivative_matches_central_difference(fp64):
+    chi, res, g, _ = fp64
+    v = bumps(11, 1.0)
+    ad = float(np.sum(g * v))
+    err = {}
+    for eps in (1e-4, 1e-5, 1e-6):
+        fd = (drag_objective(params(), chi + eps * v) - drag_objective(params(), chi - eps * v)) / (2 * eps)
+        err[eps] = abs(ad - fd) / abs(fd)
+        print(f"[gpu] directional eps {eps:.0e}: adjoint {ad:.10e} fd {fd:.10e} rel {err[eps]:.2e}", flush=True)
+    # central differences converge to the adjoint as eps^2 until round-off takes over
+    assert min(err.values()) <= 1e-6
+
+
+def test_pointwise_derivatives_match_central_difference(fp64):
+    chi, _, g, _ = fp64
+    flat = np.argsort(np.abs(g).ravel())[::-1]
+    picks = [np.unravel_index(flat[r], g.shape) for r in (0, 7, 40)]
+    # the largest gradient next to each wall, the inlet and the outlet column
+    picks.append((int(np.argmax(np.abs(g[:, 1]))), 1))
+    picks.append((int(np.argmax(np.abs(g[:, NY - 2]))), NY - 2))
+    picks.append((1, int(np.argmax(np.abs(g[1, :])))))
+    picks.append((NX - 2, int(np.argmax(np.abs(g[NX - 2, :])))))
+    eps = 1e-5
+    for (i, j) in picks:
+        e = np.zeros_like(chi)
+        e[i, j] = eps
+        fd = (drag_objective(params(), chi + e) - drag_objective(params(), chi - e)) / (2 * eps)
+        print(f"[gpu] node ({i},{j}): adjoint {g[i, j]:.10e} fd {fd:.10e}", flush=True)
+        assert abs(g[i, j] - fd) <= 1e-5 * abs(fd) + 1e-9
+
+
+def test_inlet_velocity_derivative_matches_central_difference(fp64):
+    chi, res, _, omega = fp64
+    h = 1e-6
+    jp = drag_objective(params(uo=UO + h, omega=omega), chi)
+    jm = drag_objective(params(uo=UO - h, omega=omega), chi)
+    fd = (jp - jm) / (2 * h)
+    print(f"[gpu] dJ/duo: adjoint {res.dJ_duo:.10e} fd {fd:.10e}", flush=True)
+    assert abs(res.dJ_duo - fd) <= 1e-5 * abs(fd)
+
+
+def test_fp32_adjoint_close_to_fp64(fp64):
+    chi, r64, g64, _ = fp64
+    r32, g32, _ = adjoint(params(capi.MRLBM_FP32_DEV), chi)
+    rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    print(f"[gpu] fp32 adjoint: J {r32.J:.8f} (fp64 {r64.J:.8f}) dJ/duo {r32.dJ_duo:.6e} "
+          f"(fp64 {r64.dJ_duo:.6e}) grad rel L2 {rel:.3e}", flush=True)
+    assert abs(r32.J - r64.J) <= 1e-4 * abs(r64.J)
+    assert abs(r32.dJ_duo - r64.dJ_duo) <= 1e-3 * abs(r64.dJ_duo)
+    assert rel < 1e-2
+
+
+def test_torch_autograd_function_wraps_the_adjoint(fp64):
+    torch = pytest.importorskip("torch")
+    chi0, _, _, omega = fp64
+
+    class DragCd(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, chi, uo):
+            p = params(uo=float(uo), omega=omega)
+            res, g, _ = adjoint(p, chi.detach().cpu().numpy())
+            ctx.save_for_backward(torch.from_numpy(g), torch.tensor(res.dJ_duo, dtype=torch.float64))
+            return torch.tensor(res.J, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gout):
+            g, duo = ctx.saved_tensors
+            return gout * g, gout * duo
+
+    chi = torch.tensor(chi0, dtype=torch.float64, requires_grad=True)
+    uo = torch.tensor(UO, dtype=torch.float64, requires_grad=True)
+    J = 3.0 * DragCd.apply(chi, uo)
+    J.backward()
+    v = torch.tensor(bumps(13, 1.0))
+    eps = 1e-4
+    with torch.no_grad():
+        fd = 3.0 * (DragCd.apply(chi + eps * v, uo) - DragCd.apply(chi - eps * v, uo)) / (2 * eps)
+    ad = float(torch.sum(chi.grad * v))
+    assert abs(ad - float(fd)) <= 1e-6 * abs(float(fd))
+    assert uo.grad is not None and np.isfinite(float(uo.grad))
