@@ -103,6 +103,29 @@ def cpu_baseline(nsteps=2):
                        f"{per_step:.2f} s/step")
 
 
+def adjoint_bench():
+    """The 512 x 128 drag-adjoint case in both precision forms: wall time of the checkpointed
+    forward run and of the reverse sweep (recomputation + adjoint), and the gradients'
+    agreement between the forms."""
+    from mrlbm_2d_amd import brinkman as B, capi
+    c = B.ADJ_CASE
+    chi = B.adjoint_case_field()
+    out, grads = {}, {}
+    for name, prec in (("fp64", capi.MRLBM_FP64), ("fp32", capi.MRLBM_FP32_DEV)):
+        B.run_adjoint_case(prec, chi)  # warm-up (allocations, code load)
+        res, g, _ = B.run_adjoint_case(prec, chi)
+        grads[name] = g
+        out[name] = {"J": res.J, "dJ_duo": res.dJ_duo,
+                     "forward_ms_per_step": round(res.forward_ms / c["nsteps"], 5),
+                     "adjoint_ms_per_step": round(res.adjoint_ms / c["nsteps"], 5),
+                     "adjoint_over_forward": round(res.adjoint_ms / res.forward_ms, 3)}
+        print(f"[bench] adjoint {name}: {out[name]}", file=sys.stderr, flush=True)
+    out["grad_rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(grads["fp32"] - grads["fp64"]) / np.linalg.norm(grads["fp64"]))
+    out["workload"] = ("512x128 channel, cylinder D=24 at (100,64.5), Re=100, u_max=0.05, 4000 steps, "
+                       "J = mean Cd over the last 2000, checkpoints every 200, seeded Brinkman bumps")
+    return out
+
+
 def copy_calibration(nbytes, reps):
     """A plain device copy of `nbytes` (torch's copy kernel), `reps` times: the known-size
     transfer that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of a profiled run."""
@@ -140,6 +163,8 @@ def main():
     print(f"[bench] FP64 plain form {NX}x{NY}", file=sys.stderr, flush=True)
     r64 = time_form(capi.MRLBM_FP64, args.steps, args.warmup, strip_rows=args.strip_rows)
     print(f"[bench] FP64: {r64}", file=sys.stderr, flush=True)
+    print("[bench] drag adjoint, 512x128", file=sys.stderr, flush=True)
+    adj = adjoint_bench()
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline: oracle", file=sys.stderr, flush=True)
@@ -169,6 +194,7 @@ def main():
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(r64["frac"], 4),
                               "traffic": TRAFFIC_BYTES["f64"] if default_shape else None}},
         "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3),
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -35,6 +35,16 @@ module mrlbm_iface
         integer(c_int32_t) :: box(4), n_stencil, levels
     end type
 
+    type, bind(C) :: mrlbm_adjoint_opts
+        integer(c_int32_t) :: iter_first, nsteps, window, checkpoint_interval
+        integer(c_int32_t) :: reserved(4)
+    end type
+
+    type, bind(C) :: mrlbm_adjoint_result
+        real(c_double) :: J, dJ_duo, forward_ms, adjoint_ms
+        real(c_double) :: reserved(4)
+    end type
+
     interface
         integer(c_int) function mrlbm_create(p, h) bind(C, name='mrlbm_create')
             import :: c_int, c_ptr, mrlbm_params
@@ -123,6 +133,22 @@ module mrlbm_iface
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h
             real(c_double), intent(out) :: v
+        end function
+        integer(c_int) function mrlbm_set_brinkman(h, chi) bind(C, name='mrlbm_set_brinkman')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(in) :: chi(*)
+        end function
+        subroutine mrlbm_default_adjoint_opts(o) bind(C, name='mrlbm_default_adjoint_opts')
+            import :: mrlbm_adjoint_opts
+            type(mrlbm_adjoint_opts), intent(out) :: o
+        end subroutine
+        integer(c_int) function mrlbm_adjoint_drag(h, o, dchi, res) bind(C, name='mrlbm_adjoint_drag')
+            import :: c_int, c_ptr, c_double, mrlbm_adjoint_opts, mrlbm_adjoint_result
+            type(c_ptr), value :: h
+            type(mrlbm_adjoint_opts), intent(in) :: o
+            real(c_double), intent(out) :: dchi(*)
+            type(mrlbm_adjoint_result), intent(out) :: res
         end function
         integer(c_int) function mrlbm_max_ux(h, v) bind(C, name='mrlbm_max_ux')
             import :: c_int, c_ptr, c_double

@@ -169,3 +169,25 @@ def test_torch_autograd_function_wraps_the_adjoint(fp64):
     ad = float(torch.sum(chi.grad * v))
     assert abs(ad - float(fd)) <= 1e-6 * abs(float(fd))
     assert uo.grad is not None and np.isfinite(float(uo.grad))
+
+
+def test_adjoint_512x128_case():
+    """The 512 x 128 adjoint case (D = 24 at (100, 64.5), Re = 100, 4000 steps, mean Cd over
+    the last 2000, checkpoints every 200): FP64 adjoint against a central difference along a
+    smooth direction, FP32 adjoint against FP64."""
+    from mrlbm_2d_amd import brinkman as B
+    c = B.ADJ_CASE
+    chi = B.adjoint_case_field()
+    r64, g64, _ = B.run_adjoint_case(capi.MRLBM_FP64, chi)
+    v = B.gaussian_bumps(c["nx"], c["ny"], c["xc"], c["yc"], c["radius"], 1.0, seed=9)
+    ad = float(np.sum(g64 * v))
+    eps = 1e-6
+    fd = (B.drag_objective(capi.MRLBM_FP64, chi + eps * v) - B.drag_objective(capi.MRLBM_FP64, chi - eps * v)) / (2 * eps)
+    r32, g32, _ = B.run_adjoint_case(capi.MRLBM_FP32_DEV, chi)
+    rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    print(f"[gpu] 512x128 fp64: J {r64.J:.10f} directional adjoint {ad:.10e} fd {fd:.10e}; "
+          f"adjoint/forward {r64.adjoint_ms / r64.forward_ms:.2f}; fp32: J {r32.J:.8f} grad rel L2 {rel:.3e} "
+          f"adjoint/forward {r32.adjoint_ms / r32.forward_ms:.2f}", flush=True)
+    assert abs(ad - fd) <= 1e-4 * abs(fd)
+    assert abs(r32.J - r64.J) <= 1e-4 * abs(r64.J)
+    assert rel < 1e-2
