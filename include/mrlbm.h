@@ -193,6 +193,50 @@ void mrlbm_default_adjoint_opts(mrlbm_adjoint_opts *o);
 int mrlbm_adjoint_drag(mrlbm_handle *h, const mrlbm_adjoint_opts *opts, double *dJ_dchi,
                        mrlbm_adjoint_result *result);
 
+/* Batched runs (no counterpart in the reference, which runs one case per process): many
+ * independent runs of uneven size advance together, one launch per kernel per step for
+ * all runs, the steps captured in HIP graphs.  Every run is an ordinary handle (set and
+ * read its state with the single-run calls above) that must not be stepped on its own
+ * while it belongs to the batch. */
+typedef struct mrlbm_batch mrlbm_batch;
+
+typedef struct mrlbm_batch_opts {
+    int32_t kernel;               /* bulk variant of every run (as mrlbm_params::kernel, not 1);
+                                     0: autotune over the candidate variants */
+    int32_t persistent;           /* bulk grid: 1 resident blocks looping over the tile table,
+                                     0 one block per tile, -1 autotune */
+    int32_t graph_steps;          /* steps per captured graph (even; 0 = 50) */
+    int32_t tune_steps;           /* timed steps per autotune candidate (0 = 20) */
+    int32_t reserved[4];
+} mrlbm_batch_opts;
+
+typedef struct mrlbm_batch_info {
+    int32_t nruns, kernel, persistent, grid, ntiles, graph_steps;
+    int64_t nodes;                /* sum of nx*ny over the runs */
+    int32_t ntune;                /* autotune candidates timed */
+    int32_t tune_kernel[16], tune_persistent[16];
+    double tune_ms[16];           /* mean step time of each candidate */
+} mrlbm_batch_info;
+
+void mrlbm_default_batch_opts(mrlbm_batch_opts *o);
+/* Creates nruns handles from runs[] (each must have a cylinder-free or cylinder geometry,
+ * no Brinkman field) and the batch around them; autotunes on a saved copy of the states. */
+int mrlbm_batch_create(const mrlbm_params *runs, int32_t nruns, const mrlbm_batch_opts *opts,
+                       mrlbm_batch **out);
+int mrlbm_batch_destroy(mrlbm_batch *b);
+const char *mrlbm_batch_last_error(const mrlbm_batch *b);
+/* Handle of run r (owned by the batch), or NULL. */
+mrlbm_handle *mrlbm_batch_run(mrlbm_batch *b, int32_t r);
+/* Re-runs the autotuner on the current states (they are restored afterwards). */
+int mrlbm_batch_autotune(mrlbm_batch *b);
+/* Advance every run nsteps; record_forces != 0 adds each step's Fd, Fl, Fl^2 to the runs'
+ * sums.  Synchronous. */
+int mrlbm_batch_step(mrlbm_batch *b, int32_t nsteps, int32_t record_forces);
+/* Per-run sums since the last reset: fd[r], fl[r], fl2[r], count[r] (any may be NULL). */
+int mrlbm_batch_forces(mrlbm_batch *b, double *fd_sum, double *fl_sum, double *fl2_sum, int64_t *count);
+int mrlbm_batch_reset_forces(mrlbm_batch *b);
+int mrlbm_batch_get_info(const mrlbm_batch *b, mrlbm_batch_info *info);
+
 /* Mean kernel time (ms) of the bulk step over the last mrlbm_step call, from HIP events
  * recorded on the handle's stream; 0 when timing is off. */
 int mrlbm_set_timing(mrlbm_handle *h, int32_t on);

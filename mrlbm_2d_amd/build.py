@@ -12,7 +12,7 @@ OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libmrlbm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["mrlbm_kernels.hip", "mrlbm_capi.hip", "mrlbm_adjoint.hip", "mrlbm_geometry.cpp"]
+SOURCES = ["mrlbm_kernels.hip", "mrlbm_capi.hip", "mrlbm_adjoint.hip", "mrlbm_batch.hip", "mrlbm_geometry.cpp"]
 # FP64 must not contract mul+add into FMA: the FP64 path is bit-compared with the
 # reference, which is built without FMA.  Packed FP32 instructions are disabled.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

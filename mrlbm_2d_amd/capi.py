@@ -29,6 +29,9 @@ EXPORTS = (
     "mrlbm_get_probes", "mrlbm_get_stats", "mrlbm_reset_stats", "mrlbm_max_ux",
     "mrlbm_check_finite", "mrlbm_set_timing", "mrlbm_last_step_ms",
     "mrlbm_set_brinkman", "mrlbm_default_adjoint_opts", "mrlbm_adjoint_drag",
+    "mrlbm_default_batch_opts", "mrlbm_batch_create", "mrlbm_batch_destroy", "mrlbm_batch_last_error",
+    "mrlbm_batch_run", "mrlbm_batch_autotune", "mrlbm_batch_step", "mrlbm_batch_forces",
+    "mrlbm_batch_reset_forces", "mrlbm_batch_get_info",
 )
 
 
@@ -77,6 +80,21 @@ class AdjointResult(C.Structure):
     _fields_ = [
         ("J", C.c_double), ("dJ_duo", C.c_double), ("forward_ms", C.c_double),
         ("adjoint_ms", C.c_double), ("reserved", C.c_double * 4),
+    ]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int32), ("persistent", C.c_int32), ("graph_steps", C.c_int32),
+        ("tune_steps", C.c_int32), ("reserved", C.c_int32 * 4),
+    ]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [
+        ("nruns", C.c_int32), ("kernel", C.c_int32), ("persistent", C.c_int32), ("grid", C.c_int32),
+        ("ntiles", C.c_int32), ("graph_steps", C.c_int32), ("nodes", C.c_int64), ("ntune", C.c_int32),
+        ("tune_kernel", C.c_int32 * 16), ("tune_persistent", C.c_int32 * 16), ("tune_ms", C.c_double * 16),
     ]
 
 
@@ -131,6 +149,19 @@ def load(path: str = LIB_PATH):
     L.mrlbm_default_adjoint_opts.argtypes = [C.POINTER(AdjointOpts)]
     L.mrlbm_default_adjoint_opts.restype = None
     L.mrlbm_adjoint_drag.argtypes = [vp, C.POINTER(AdjointOpts), dp, C.POINTER(AdjointResult)]
+    L.mrlbm_default_batch_opts.argtypes = [C.POINTER(BatchOpts)]
+    L.mrlbm_default_batch_opts.restype = None
+    L.mrlbm_batch_create.argtypes = [C.POINTER(Params), C.c_int32, C.POINTER(BatchOpts), C.POINTER(vp)]
+    L.mrlbm_batch_destroy.argtypes = [vp]
+    L.mrlbm_batch_last_error.argtypes = [vp]
+    L.mrlbm_batch_last_error.restype = C.c_char_p
+    L.mrlbm_batch_run.argtypes = [vp, C.c_int32]
+    L.mrlbm_batch_run.restype = vp
+    L.mrlbm_batch_autotune.argtypes = [vp]
+    L.mrlbm_batch_step.argtypes = [vp, C.c_int32, C.c_int32]
+    L.mrlbm_batch_forces.argtypes = [vp, dp, dp, dp, C.POINTER(C.c_int64)]
+    L.mrlbm_batch_reset_forces.argtypes = [vp]
+    L.mrlbm_batch_get_info.argtypes = [vp, C.POINTER(BatchInfo)]
     _lib = L
     return L
 
@@ -171,12 +202,16 @@ def _fort(a, shape):
 class Lattice:
     """One run on the GPU behind the C-ABI (handle + device state)."""
 
-    def __init__(self, params: Params):
+    def __init__(self, params: Params, _handle=None):
         self._L = load()
-        h = C.c_void_p()
-        rc = self._L.mrlbm_create(C.byref(params), C.byref(h))
-        if rc != 0:
-            raise MrlbmError(rc, self._L.mrlbm_last_error(None).decode())
+        self._owned = _handle is None
+        if _handle is None:
+            h = C.c_void_p()
+            rc = self._L.mrlbm_create(C.byref(params), C.byref(h))
+            if rc != 0:
+                raise MrlbmError(rc, self._L.mrlbm_last_error(None).decode())
+        else:
+            h = C.c_void_p(_handle)
         self._h = h
         self.params = params
         info = Info()
@@ -191,7 +226,8 @@ class Lattice:
 
     def close(self):
         if getattr(self, "_h", None):
-            self._L.mrlbm_destroy(self._h)
+            if self._owned:
+                self._L.mrlbm_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -326,3 +362,75 @@ class Lattice:
         v = C.c_double()
         self._check(self._L.mrlbm_last_step_ms(self._h, C.byref(v)))
         return v.value
+
+
+def default_batch_opts() -> BatchOpts:
+    o = BatchOpts()
+    load().mrlbm_default_batch_opts(C.byref(o))
+    return o
+
+
+class Batch:
+    """Independent runs advanced together (mrlbm_batch_*).  run(r) is a Lattice view of run r
+    for setting and reading its state; step the runs only through the batch."""
+
+    def __init__(self, params_list, opts: BatchOpts | None = None):
+        self._L = load()
+        arr = (Params * len(params_list))(*params_list)
+        o = opts if opts is not None else default_batch_opts()
+        b = C.c_void_p()
+        rc = self._L.mrlbm_batch_create(arr, len(params_list), C.byref(o), C.byref(b))
+        if rc != 0:
+            raise MrlbmError(rc, self._L.mrlbm_batch_last_error(None).decode())
+        self._b = b
+        self.params = list(params_list)
+        self.nruns = len(params_list)
+        self._views = [Lattice(p, _handle=self._L.mrlbm_batch_run(b, r)) for r, p in enumerate(params_list)]
+
+    def _check(self, rc):
+        if rc != 0:
+            raise MrlbmError(rc, self._L.mrlbm_batch_last_error(self._b).decode())
+
+    def run(self, r: int) -> Lattice:
+        return self._views[r]
+
+    def step(self, nsteps: int, record_forces: bool = False):
+        self._check(self._L.mrlbm_batch_step(self._b, int(nsteps), 1 if record_forces else 0))
+
+    def autotune(self):
+        self._check(self._L.mrlbm_batch_autotune(self._b))
+
+    def forces(self):
+        fd, fl, fl2 = (np.zeros(self.nruns) for _ in range(3))
+        n = np.zeros(self.nruns, np.int64)
+        self._check(self._L.mrlbm_batch_forces(self._b, _dp(fd), _dp(fl), _dp(fl2),
+                                                n.ctypes.data_as(C.POINTER(C.c_int64))))
+        return fd, fl, fl2, n
+
+    def reset_forces(self):
+        self._check(self._L.mrlbm_batch_reset_forces(self._b))
+
+    @property
+    def info(self) -> BatchInfo:
+        i = BatchInfo()
+        self._check(self._L.mrlbm_batch_get_info(self._b, C.byref(i)))
+        return i
+
+    def close(self):
+        if getattr(self, "_b", None):
+            for v in self._views:
+                v.close()
+            self._L.mrlbm_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -194,6 +194,61 @@ static void free_all(mrlbm_handle *h)
     if (h->st2) (void)hipStreamDestroy(h->st2);
 }
 
+int mrlbm::set_bulk_shape(mrlbm_handle *h, int kernel_in, int strip_rows, std::string &why)
+{
+    // 0 auto picks the variant measured fastest at 8192 x 2048 on MI355X
+    const int kernel = kernel_in == 0 ? (h->f32 ? 304 : 8) : kernel_in;
+    const bool aligned = kernel == 304 || kernel == 308 || kernel == 316;
+    int H = strip_rows;
+    if (kernel == 4 || kernel == 8 || kernel == 16) {
+        H = kernel; // unrolled strip of that many rows
+    } else if (aligned) {
+        H = kernel - 300; // aligned 64-column waves, unrolled strip of kernel-300 rows
+    } else if (kernel != 1 && kernel != 2) {
+        why = "kernel must be 0, 1, 2, 4, 8, 16, 304, 308 or 316";
+        return MRLBM_E_ARG;
+    }
+    // waves of 64 (aligned) or 62 written columns, strips of rows
+    const int ncg = aligned ? (h->nx + 63) / 64 : (h->nx + 61) / 62;
+    if (H <= 0) {
+        // aim for >= 8 bulk waves per SIMD slot of the 256 CUs, strips of 8..64 rows
+        const long long target = 256LL * 4 * 8;
+        H = 64;
+        while (H > 8 && (long long)ncg * ((h->ny - 2 + H - 1) / H) < target) H /= 2;
+    }
+    if (H > 64) {
+        why = "strip_rows must be <= 64";
+        return MRLBM_E_ARG;
+    }
+    h->p.kernel = kernel_in;
+    h->p.strip_rows = strip_rows;
+    h->variant = (kernel == 1 || kernel == 2) ? 0 : kernel;
+    h->ncg = ncg;
+    h->ncb = (ncg + 3) / 4;
+    h->strip_rows = H;
+    h->nstrips = (h->ny - 2 + H - 1) / H; // interior rows 1..ny-2; the edge kernel does the ring
+    if (h->bt.bw > 0) { // per box column and bulk strip: 64-bit row masks of boundary, solid nodes
+        std::vector<unsigned long long> bits((size_t)h->bt.bw * h->nstrips * 2, 0ull);
+        for (int bi = 0; bi < h->bt.bw; bi++)
+            for (int bj = 0; bj < h->bt.bh; bj++) {
+                const int j = h->bt.by0 + bj;
+                if (j < 1 || j > h->ny - 2) continue;
+                const int st = (j - 1) / h->strip_rows, t = (j - 1) % h->strip_rows;
+                const uint8_t c = h->bt.boxcls[(size_t)bj * h->bt.bw + bi];
+                if (c & 1) bits[((size_t)bi * h->nstrips + st) * 2] |= 1ull << t;
+                if (c & 2) bits[((size_t)bi * h->nstrips + st) * 2 + 1] |= 1ull << t;
+            }
+        if (h->d_boxbits) (void)hipFree(h->d_boxbits);
+        h->d_boxbits = nullptr;
+        if (hipMalloc(&h->d_boxbits, bits.size() * sizeof(unsigned long long)) != hipSuccess ||
+            hipMemcpy(h->d_boxbits, bits.data(), bits.size() * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess) {
+            why = "box mask upload failed";
+            return MRLBM_E_HIP;
+        }
+    }
+    return 0;
+}
+
 extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
 {
     g_create_error.clear();
@@ -233,33 +288,6 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
     else h->nu = uref * (double)(p->ny - 1) / p->Re;
     h->omega = p->omega > 0 ? p->omega : 1.0 / ((3.0 * h->nu) + (1.0 / 2.0));
     if (p->omega > 0) h->nu = (1.0 / p->omega - 0.5) / 3.0;
-    // bulk variant: 0 auto picks the fastest measured at 8192 x 2048 on MI355X
-    int kernel = p->kernel;
-    if (kernel == 0) kernel = h->f32 ? 304 : 8;
-    const bool aligned = kernel == 304 || kernel == 308 || kernel == 316;
-    // bulk decomposition: waves of 64 (aligned) or 62 written columns, strips of rows
-    h->ncg = aligned ? (h->nx + 63) / 64 : (h->nx + 61) / 62;
-    h->ncb = (h->ncg + 3) / 4;
-    int H = p->strip_rows;
-    if (H <= 0) {
-        // aim for >= 8 bulk waves per SIMD slot of the 256 CUs, strips of 8..64 rows
-        const long long target = 256LL * 4 * 8;
-        H = 64;
-        while (H > 8 && (long long)h->ncg * ((h->ny - 2 + H - 1) / H) < target) H /= 2;
-    }
-    if (kernel == 4 || kernel == 8 || kernel == 16) {
-        h->variant = kernel; // unrolled strip of that many rows
-        H = kernel;
-    } else if (aligned) {
-        h->variant = kernel; // aligned 64-column waves, unrolled strip of kernel-300 rows
-        H = kernel - 300;
-    } else if (kernel == 1 || kernel == 2) {
-        h->variant = 0;      // 1: one node per thread (a.simple), 2: row loop over H rows
-    } else {
-        return fail(MRLBM_E_ARG, "kernel must be 0, 1, 2, 4, 8, 16, 304, 308 or 316");
-    }
-    h->strip_rows = H;
-    h->nstrips = (h->ny - 2 + H - 1) / H; // interior rows 1..ny-2; the edge kernel does the ring
     if (hipStreamCreate(&h->st) != hipSuccess || hipStreamCreate(&h->st2) != hipSuccess)
         return fail(MRLBM_E_HIP, "hipStreamCreate failed");
     if (hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess ||
@@ -270,21 +298,10 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
         return fail(MRLBM_E_HIP, "hipEventRecord failed");
     const int rc = h->f32 ? create_typed<float, float>(h) : create_typed<double, double>(h);
     if (rc) return fail(rc, "");
-    if (h->bt.bw > 0) { // per box column and bulk strip: 64-bit row masks (strips are <= 64 rows)
-        if (h->strip_rows > 64) return fail(MRLBM_E_ARG, "strip_rows must be <= 64");
-        std::vector<unsigned long long> bits((size_t)h->bt.bw * h->nstrips * 2, 0ull);
-        for (int bi = 0; bi < h->bt.bw; bi++)
-            for (int bj = 0; bj < h->bt.bh; bj++) {
-                const int j = h->bt.by0 + bj;
-                if (j < 1 || j > h->ny - 2) continue;
-                const int st = (j - 1) / h->strip_rows, t = (j - 1) % h->strip_rows;
-                const uint8_t c = h->bt.boxcls[(size_t)bj * h->bt.bw + bi];
-                if (c & 1) bits[((size_t)bi * h->nstrips + st) * 2] |= 1ull << t;
-                if (c & 2) bits[((size_t)bi * h->nstrips + st) * 2 + 1] |= 1ull << t;
-            }
-        if (hipMalloc(&h->d_boxbits, bits.size() * sizeof(unsigned long long)) != hipSuccess ||
-            hipMemcpy(h->d_boxbits, bits.data(), bits.size() * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess)
-            return fail(MRLBM_E_HIP, "box mask upload failed");
+    {
+        std::string why;
+        const int rc2 = set_bulk_shape(h, p->kernel, p->strip_rows, why);
+        if (rc2) return fail(rc2, why);
     }
     *out = h;
     return MRLBM_OK;

@@ -12,6 +12,9 @@
 namespace mrlbm {
 struct AdjointWork;
 void adjoint_free(mrlbm_handle *h);
+// Bulk decomposition for a kernel choice (see mrlbm_params::kernel): variant, strip height,
+// wave columns, strips, and the per-strip boundary/solid row masks of the cylinder box.
+int set_bulk_shape(mrlbm_handle *h, int kernel, int strip_rows, std::string &why);
 } // namespace mrlbm
 
 using namespace mrlbm;

@@ -53,6 +53,7 @@ template <class S, class T> struct StepArgs {
     const double *inlet_prev;  // previous window step's inlet rho, or null
     double inlet_prev_c;       // its mean counter
     double *rho_inf_mean;
+    double *facc;              // batched runs: running sums of Fd, Fl, Fl^2 and the step count
     const T *chi;              // Brinkman solid fraction (per-node bulk kernel only), or null
     BoundaryDev b;
 };
@@ -74,5 +75,13 @@ template <class S, class T>
 hipError_t launch_ghost_from_pops(const StepArgs<S, T> &a, T *ghost, hipStream_t st);
 template <class S, class T>
 hipError_t launch_stencil_moments(const StepArgs<S, T> &a, hipStream_t st);
+// Batched runs: args[r] per run in device memory; tiles[t] = (run, bulk block of that run).
+template <class S, class T>
+hipError_t launch_bulk_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int variant, int grid,
+                             hipStream_t st);
+template <class S, class T>
+hipError_t launch_side_batch(const StepArgs<S, T> *args, int nruns, int max_ring, int max_nU, hipStream_t st);
+// Resident blocks of the batched bulk kernel of a variant on the whole device (0 on error).
+template <class S, class T> int bulk_batch_resident_blocks(int variant);
 
 } // namespace mrlbm

@@ -302,9 +302,8 @@ template <class S, class T> __device__ void bulk_simple(const StepArgs<S, T> &a,
 
 // Edge nodes (the domain's outer ring): inlet, outlet, channel walls or periodic wraps,
 // stale populations, in the reference's BC order (2d_mrlbm.f90:247-351).
-template <class S, class T> __global__ void __launch_bounds__(256) edge_kernel(StepArgs<S, T> a)
+template <class S, class T> __device__ void edge_body(const StepArgs<S, T> &a, int r)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int nx = a.nx, ny = a.ny;
     if (r >= 2 * nx + 2 * (ny - 2)) return;
     int c, j;
@@ -317,11 +316,15 @@ template <class S, class T> __global__ void __launch_bounds__(256) edge_kernel(S
     node_update(a, c, j, f, true);
 }
 
-template <class S, class T> __global__ void __launch_bounds__(256) stencil_moments_kernel(StepArgs<S, T> a)
+template <class S, class T> __global__ void __launch_bounds__(256) edge_kernel(StepArgs<S, T> a)
+{
+    edge_body(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <class S, class T> __device__ void stencil_moments_body(const StepArgs<S, T> &a, int u)
 {
     using F = typename FormOf<T>::F;
     const BoundaryDev &b = a.b;
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= b.nU) return;
     T *U = (T *)b.U;
     const int nU = b.nU;
@@ -338,9 +341,14 @@ template <class S, class T> __global__ void __launch_bounds__(256) stencil_momen
     }
 }
 
+template <class S, class T> __global__ void __launch_bounds__(256) stencil_moments_kernel(StepArgs<S, T> a)
+{
+    stencil_moments_body(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Wall BC, collision, store, momentum exchange and surface-pressure feed of the boundary
 // nodes (one workgroup), after stencil_moments_kernel of the same step.
-template <class S, class T> __global__ void __launch_bounds__(256) boundary_kernel(StepArgs<S, T> a)
+template <class S, class T> __device__ void boundary_body(const StepArgs<S, T> &a)
 {
     using F = typename FormOf<T>::F;
     const BoundaryDev &b = a.b;
@@ -440,6 +448,12 @@ template <class S, class T> __global__ void __launch_bounds__(256) boundary_kern
         if (tid == 0) {
             rec[0] = sx;
             rec[1] = sy;
+            if (a.facc) {
+                a.facc[0] += sx;
+                a.facc[1] += sy;
+                a.facc[2] += sy * sy;
+                a.facc[3] += 1.0;
+            }
         }
         if (a.record_p) { // theta-sorted (:602-604) and Gaussian-smoothed (:722-741)
             double *ps = rec + 14, *pf = rec + 14 + b.nbct;
@@ -470,13 +484,58 @@ template <class S, class T> __global__ void __launch_bounds__(256) boundary_kern
     }
 }
 
+template <class S, class T> __global__ void __launch_bounds__(256) boundary_kernel(StepArgs<S, T> a)
+{
+    boundary_body(a);
+}
+
 // H: 0 row loop over a runtime strip height, 4/8/16 unrolled strip of H rows (62 written
 // columns per wave), 304/308/316 aligned 64-column waves with an unrolled strip of H-300 rows.
+template <class S, class T, bool REC, int H> __device__ __forceinline__ void bulk_body(const StepArgs<S, T> &a, int bb)
+{
+    if constexpr (H >= 300) bulk_march_a<S, T, REC, H - 300>(a, bb);
+    else if constexpr (H == 0) bulk_march<S, T, REC>(a, bb);
+    else bulk_march_u<S, T, REC, H>(a, bb);
+}
+
 template <class S, class T, bool REC, int H> __global__ void __launch_bounds__(256) bulk_kernel(StepArgs<S, T> a)
 {
-    if constexpr (H >= 300) bulk_march_a<S, T, REC, H - 300>(a, blockIdx.x);
-    else if constexpr (H == 0) bulk_march<S, T, REC>(a, blockIdx.x);
-    else bulk_march_u<S, T, REC, H>(a, blockIdx.x);
+    bulk_body<S, T, REC, H>(a, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------------
+// Batched runs: per-run argument blocks in device memory, one launch per kernel for all
+// runs.  The bulk walks a flattened tile table (run, block of that run) with a persistent
+// grid; the edge and stencil kernels take the run from blockIdx.y, the boundary workgroup
+// of run r is block r.
+template <class S, class T, int H>
+__global__ void __launch_bounds__(256) bulk_batch_kernel(const StepArgs<S, T> *args, const int2 *tiles, int ntiles)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        const StepArgs<S, T> a = args[tl.x];
+        bulk_body<S, T, false, H>(a, tl.y);
+    }
+}
+
+template <class S, class T> __global__ void __launch_bounds__(256) edge_batch_kernel(const StepArgs<S, T> *args)
+{
+    const StepArgs<S, T> a = args[blockIdx.y];
+    edge_body(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <class S, class T> __global__ void __launch_bounds__(256) stencil_batch_kernel(const StepArgs<S, T> *args)
+{
+    const StepArgs<S, T> &a = args[blockIdx.y];
+    if (!a.has_boundary) return;
+    stencil_moments_body(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <class S, class T> __global__ void __launch_bounds__(256) boundary_batch_kernel(const StepArgs<S, T> *args)
+{
+    const StepArgs<S, T> &a = args[blockIdx.x];
+    if (!a.has_boundary) return;
+    boundary_body(a);
 }
 
 template <class S, class T> __global__ void __launch_bounds__(256) bulk_simple_kernel(StepArgs<S, T> a)
@@ -574,6 +633,51 @@ template <class S, class T> hipError_t launch_boundary(const StepArgs<S, T> &a, 
     hipLaunchKernelGGL((boundary_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
     return hipGetLastError();
 }
+template <class S, class T>
+hipError_t launch_bulk_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int variant, int grid, hipStream_t st)
+{
+    const dim3 g(grid), b(256);
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((bulk_batch_kernel<S, T, 0>), g, b, 0, st, args, tiles, ntiles); break;
+    case 4: hipLaunchKernelGGL((bulk_batch_kernel<S, T, 4>), g, b, 0, st, args, tiles, ntiles); break;
+    case 8: hipLaunchKernelGGL((bulk_batch_kernel<S, T, 8>), g, b, 0, st, args, tiles, ntiles); break;
+    case 16: hipLaunchKernelGGL((bulk_batch_kernel<S, T, 16>), g, b, 0, st, args, tiles, ntiles); break;
+    case 304: hipLaunchKernelGGL((bulk_batch_kernel<S, T, 304>), g, b, 0, st, args, tiles, ntiles); break;
+    case 308: hipLaunchKernelGGL((bulk_batch_kernel<S, T, 308>), g, b, 0, st, args, tiles, ntiles); break;
+    case 316: hipLaunchKernelGGL((bulk_batch_kernel<S, T, 316>), g, b, 0, st, args, tiles, ntiles); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template <class S, class T> int bulk_batch_resident_blocks(int variant)
+{
+    int per_cu = 0, dev = 0, ncu = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (variant) {
+    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bulk_batch_kernel<S, T, 0>, 256, 0); break;
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bulk_batch_kernel<S, T, 4>, 256, 0); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bulk_batch_kernel<S, T, 8>, 256, 0); break;
+    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bulk_batch_kernel<S, T, 16>, 256, 0); break;
+    case 304: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bulk_batch_kernel<S, T, 304>, 256, 0); break;
+    case 308: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bulk_batch_kernel<S, T, 308>, 256, 0); break;
+    case 316: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bulk_batch_kernel<S, T, 316>, 256, 0); break;
+    default: break;
+    }
+    if (e != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return per_cu * ncu;
+}
+template <class S, class T>
+hipError_t launch_side_batch(const StepArgs<S, T> *args, int nruns, int max_ring, int max_nU, hipStream_t st)
+{
+    hipLaunchKernelGGL((edge_batch_kernel<S, T>), dim3((max_ring + 255) / 256, nruns), dim3(256), 0, st, args);
+    if (max_nU > 0) {
+        hipLaunchKernelGGL((stencil_batch_kernel<S, T>), dim3((max_nU + 255) / 256, nruns), dim3(256), 0, st, args);
+        hipLaunchKernelGGL((boundary_batch_kernel<S, T>), dim3(nruns), dim3(256), 0, st, args);
+    }
+    return hipGetLastError();
+}
 template <class S, class T> hipError_t launch_stencil_moments(const StepArgs<S, T> &a, hipStream_t st)
 {
     hipLaunchKernelGGL((stencil_moments_kernel<S, T>), dim3((a.b.nU + 255) / 256), dim3(256), 0, st, a);
@@ -608,6 +712,9 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_boundary<S, T>(const StepArgs<S, T> &, hipStream_t);              \
     template hipError_t launch_edges<S, T>(const StepArgs<S, T> &, hipStream_t);                 \
     template hipError_t launch_stencil_moments<S, T>(const StepArgs<S, T> &, hipStream_t);       \
+    template hipError_t launch_bulk_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, int, hipStream_t); \
+    template hipError_t launch_side_batch<S, T>(const StepArgs<S, T> *, int, int, int, hipStream_t); \
+    template int bulk_batch_resident_blocks<S, T>(int);                                          \
     template hipError_t launch_inlet_fold<S, T>(const StepArgs<S, T> &, hipStream_t);            \
     template hipError_t launch_get_populations<S, T>(const StepArgs<S, T> &, double *, hipStream_t); \
     template hipError_t launch_ghost_from_state<S, T>(const StepArgs<S, T> &, T *, hipStream_t);  \

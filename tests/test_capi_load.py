@@ -10,7 +10,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     txt = open(os.path.join(REPO, "include", "mrlbm.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*(mrlbm_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*|mrlbm_handle \*)\s*(mrlbm_\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_header_and_binding_agree():
