@@ -126,6 +126,61 @@ def adjoint_bench():
     return out
 
 
+def batch_bench(nruns=256, steps=20000, window=10000, seed=0, sample=(0, 1, 2)):
+    """256 independent FP32 cylinder runs of uneven size in one batch: aggregate MLUPS over
+    `steps` steps of every run, per-run mean Cd over the last `window` steps, and the same Cd
+    from a few of the runs stepped alone."""
+    from mrlbm_2d_amd import batch_case as BC, capi
+    ps = BC.uneven_runs(nruns, seed=seed)
+    t0 = time.perf_counter()
+    b = capi.Batch(ps)
+    setup_s = time.perf_counter() - t0
+    info = b.info
+    tune = [{"kernel": info.tune_kernel[i], "persistent": info.tune_persistent[i],
+             "ms_per_step": round(info.tune_ms[i], 4)} for i in range(info.ntune)]
+    print(f"[bench] batch: {nruns} runs, {info.nodes} nodes, kernel {info.kernel} persistent "
+          f"{info.persistent}, setup+autotune {setup_s:.1f} s", file=sys.stderr, flush=True)
+    for r in range(nruns):
+        BC.seed_state(b.run(r), seed, r)
+    t0 = time.perf_counter()
+    b.step(steps - window)
+    b.step(window, record_forces=True)
+    wall = time.perf_counter() - t0
+    fd, fl, fl2, cnt = b.forces()
+    r_cyl = [b.run(r).info.r_cyl for r in range(nruns)]
+    omega = [b.run(r).info.omega for r in range(nruns)]
+    cd = [BC.cd_of(ps[r], r_cyl[r], fd[r] / cnt[r]) for r in range(nruns)]
+    b.close()
+    print(f"[bench] batch: {steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
+    cmp = []
+    for r in sample:
+        with capi.Lattice(ps[r]) as lat:
+            BC.seed_state(lat, seed, r)
+            lat.step(2, steps - window)
+            o = capi.default_step_opts()
+            o.post_process = 1
+            o.statsbegin = 2 + steps - window
+            o.statsend = 1 + steps
+            lat.step(2 + steps - window, window, o)
+            _, sfd, _ = lat.forces()
+            s = 0.0
+            for v in sfd:
+                s += v
+            cd1 = BC.cd_of(ps[r], lat.info.r_cyl, s / len(sfd))
+        cmp.append({"run": r, "nx": ps[r].nx, "Re": round(ps[r].Re, 2), "cd_batch": cd[r], "cd_single": cd1})
+    finite = [c for c in cd if np.isfinite(c)]
+    diverged = [{"run": r, "nx": ps[r].nx, "D": round(2 * ps[r].r_c, 2), "Re": round(ps[r].Re, 1),
+                 "omega": round(omega[r], 4)} for r in range(nruns) if not np.isfinite(cd[r])]
+    return {"workload": f"{nruns} FP32 cylinder runs, nx in 256..2048 step 64, ny = nx/4, D in [ny/8, ny/4] "
+                        f"at (nx/5, ny/2+0.5), Re in [40, 300], u_max = 0.05, seed {seed}; {steps} steps, "
+                        f"Cd over the last {window}",
+            "nodes": int(info.nodes), "value": round(info.nodes * steps / wall / 1e6, 1), "unit": "MLUPS",
+            "wall_s": round(wall, 3), "kernel": info.kernel, "persistent": info.persistent,
+            "graph_steps": info.graph_steps, "autotune": tune, "setup_s": round(setup_s, 2),
+            "cd_finite_runs": len(finite), "cd_min": min(finite) if finite else None,
+            "cd_max": max(finite) if finite else None, "diverged": diverged, "single_run_check": cmp}
+
+
 def copy_calibration(nbytes, reps):
     """A plain device copy of `nbytes` (torch's copy kernel), `reps` times: the known-size
     transfer that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of a profiled run."""
@@ -144,6 +199,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (for profiler counter scaling)")
@@ -165,6 +221,10 @@ def main():
     print(f"[bench] FP64: {r64}", file=sys.stderr, flush=True)
     print("[bench] drag adjoint, 512x128", file=sys.stderr, flush=True)
     adj = adjoint_bench()
+    batch = None
+    if not args.no_batch:
+        print("[bench] batched runs", file=sys.stderr, flush=True)
+        batch = batch_bench()
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline: oracle", file=sys.stderr, flush=True)
@@ -195,6 +255,7 @@ def main():
                               "traffic": TRAFFIC_BYTES["f64"] if default_shape else None}},
         "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3),
         "adjoint": adj,
+        "batched": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
