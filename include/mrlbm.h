@@ -61,7 +61,11 @@ typedef struct mrlbm_params {
                                      of that many rows with 62 written columns per wave, 304/308/316
                                      aligned 64-column waves with an unrolled strip of kernel-300 rows */
     int32_t block_order;          /* bulk blocks: 0 plain, 1 XCD-aware (vertical neighbours share an L2) */
-    int32_t reserved[5];
+    int32_t time_blocking;        /* 0: one bulk pass per step; 2/4/8: two steps per bulk pass over the
+                                     plain interior (strips of that many rows), the edges and the
+                                     cylinder band in two ordinary sub-steps; -1 (default) auto:
+                                     4 in FP32, 2 in FP64.  Same bits either way */
+    int32_t reserved[4];
 } mrlbm_params;
 
 /* What one call to mrlbm_step records (2d_mrlbm.f90:355-502). */

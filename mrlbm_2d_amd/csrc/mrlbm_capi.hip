@@ -35,6 +35,7 @@ extern "C" void mrlbm_default_params(mrlbm_params *p)
     p->incomp_bulk = 0;
     p->precision = MRLBM_FP64;
     p->label_policy = MRLBM_LABEL_REFERENCE;
+    p->time_blocking = -1;
 }
 
 extern "C" void mrlbm_default_step_opts(mrlbm_step_opts *o)
@@ -181,7 +182,7 @@ static void free_all(mrlbm_handle *h)
 {
     adjoint_free(h);
     for (void *p : {h->S[0], h->S[1], h->pops, h->ghost, h->uin, (void *)h->d_boxbits, (void *)h->d_rec, (void *)h->d_inlet[0],
-                    (void *)h->d_inlet[1], (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red, h->chi})
+                    (void *)h->d_inlet[1], (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red, h->chi, h->S2})
         if (p) (void)hipFree(p);
     for (void *p : h->allocs)
         if (p) (void)hipFree(p);
@@ -249,6 +250,46 @@ int mrlbm::set_bulk_shape(mrlbm_handle *h, int kernel_in, int strip_rows, std::s
     return 0;
 }
 
+// Two-step pass: the scratch state and the band rectangles.  Sub-step 1 (t -> t+1 into the
+// scratch) covers the two layers inside the ring and the cylinder box grown to hold every
+// node the wall kernels and sub-step 2 read; sub-step 2 (t+1 -> t+2) covers the layer
+// inside the ring and the cylinder box grown by one.  The ring and the boundary nodes are
+// done by the edge and wall kernels in each sub-step; the two-step bulk does the rest.
+static int setup_time_blocking(mrlbm_handle *h, int H)
+{
+    const int nx = h->nx, ny = h->ny;
+    HIPCHK(hipMalloc(&h->S2, (size_t)6 * h->plane * (h->f32 ? sizeof(float) : sizeof(double))));
+    h->tb_H = H;
+    h->nstrips2 = (ny - 4 + H - 1) / H; // output rows 2..ny-3
+    h->ncg2 = (nx + 63) / 64;            // aligned 64-column waves whatever the single-step variant
+    h->ncb2 = (h->ncg2 + 3) / 4;
+    auto put = [](int r[4], int x0, int y0, int x1, int y1) { r[0] = x0; r[1] = y0; r[2] = x1 - x0 + 1; r[3] = y1 - y0 + 1; };
+    put(h->rect1[0], 1, 1, nx - 2, 2);
+    put(h->rect1[1], 1, ny - 3, nx - 2, ny - 2);
+    put(h->rect1[2], 1, 3, 2, ny - 4);
+    put(h->rect1[3], nx - 3, 3, nx - 2, ny - 4);
+    put(h->rect2[0], 1, 1, nx - 2, 1);
+    put(h->rect2[1], 1, ny - 2, nx - 2, ny - 2);
+    put(h->rect2[2], 1, 2, 1, ny - 3);
+    put(h->rect2[3], nx - 2, 2, nx - 2, ny - 3);
+    h->nrect1 = h->nrect2 = 4;
+    if (h->bt.bw > 0) {
+        const BoundaryTables &t = h->bt;
+        int x0 = t.bx0 - 2, x1 = t.bx0 + t.bw + 1, y0 = t.by0 - 2, y1 = t.by0 + t.bh + 1;
+        for (int u = 0; u < t.nU; u++) {
+            x0 = std::min(x0, t.uij[2 * u] - 1);
+            x1 = std::max(x1, t.uij[2 * u] + 1);
+            y0 = std::min(y0, t.uij[2 * u + 1] - 1);
+            y1 = std::max(y1, t.uij[2 * u + 1] + 1);
+        }
+        put(h->rect1[4], std::max(x0, 1), std::max(y0, 1), std::min(x1, nx - 2), std::min(y1, ny - 2));
+        put(h->rect2[4], std::max(t.bx0 - 1, 1), std::max(t.by0 - 1, 1), std::min(t.bx0 + t.bw, nx - 2),
+            std::min(t.by0 + t.bh, ny - 2));
+        h->nrect1 = h->nrect2 = 5;
+    }
+    return 0;
+}
+
 extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
 {
     g_create_error.clear();
@@ -302,6 +343,15 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
         std::string why;
         const int rc2 = set_bulk_shape(h, p->kernel, p->strip_rows, why);
         if (rc2) return fail(rc2, why);
+    }
+    {
+        // -1 auto: the strip height measured fastest at 8192 x 2048 on MI355X
+        const int tb = p->time_blocking < 0 ? (h->f32 ? 4 : 2) : p->time_blocking;
+        if (tb != 0 && tb != 2 && tb != 4 && tb != 8) return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4 or 8");
+        if (tb && h->nx >= 16 && h->ny >= 16) {
+            const int rc3 = setup_time_blocking(h, tb);
+            if (rc3) return fail(rc3, "");
+        }
     }
     *out = h;
     return MRLBM_OK;
@@ -527,20 +577,103 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
     const long long N = (long long)h->nx * h->ny;
     const int simple_blocks = (int)((N + 255) / 256);
     std::vector<int32_t> rec_iters;
-    if (h->timing) HIPCHK(hipEventRecord(h->e0, h->st));
-    for (int s = 0; s < nsteps; s++) {
-        const int iter = iter_first + s;
-        const bool window = opts.post_process && iter >= opts.statsbegin && iter <= opts.statsend;
+    auto is_window = [&](int iter) { return opts.post_process && iter >= opts.statsbegin && iter <= opts.statsend; };
+    auto plot_step = [&](int iter) { return opts.post_process && opts.iplot > 0 && iter % opts.iplot == 0; };
+    auto sample_step = [&](int iter) {
+        return is_window(iter) && opts.stats_interval > 0 && (iter - opts.statsbegin) % opts.stats_interval == 0;
+    };
+    // kernel arguments of iteration iter reading state src and writing dst
+    auto step_args = [&](int iter, const void *src, void *dst, int rec_slot) {
+        const bool window = is_window(iter);
         auto a = base_args<S, T>(h);
-        a.zero_solid = (opts.post_process && opts.iplot > 0 && iter % opts.iplot == 0) ? 1 : 0;
-        a.sample = (window && opts.stats_interval > 0 && (iter - opts.statsbegin) % opts.stats_interval == 0) ? 1 : 0;
-        a.rec_slot = window ? (int)rec_iters.size() : -1;
+        a.src = (const S *)src;
+        a.dst = (S *)dst;
+        a.zero_solid = plot_step(iter) ? 1 : 0;
+        a.sample = sample_step(iter) ? 1 : 0;
+        a.rec_slot = window ? rec_slot : -1;
         a.record_p = window && opts.record_surface_p ? 1 : 0;
         a.nprobe = opts.nprobe;
         for (int p = 0; p < 4; p++) a.probe_node[p] = probe_node[p];
         a.inlet_out = window ? h->d_inlet[iter & 1] : nullptr;
         a.inlet_prev = (h->prev_inlet_valid && h->has_b) ? h->d_inlet[h->prev_inlet_buf] : nullptr;
         a.inlet_prev_c = h->prev_inlet_c;
+        return a;
+    };
+    // curved wall (or, without a cylinder, the inlet-density fold) of one iteration
+    auto side_wall = [&](const StepArgs<S, T> &a) -> hipError_t {
+        if (h->has_b) return launch_boundary<S, T>(a, h->st2);
+        if (h->prev_inlet_valid) {
+            auto f = a;
+            f.inlet_prev = h->d_inlet[h->prev_inlet_buf];
+            return launch_inlet_fold<S, T>(f, h->st2);
+        }
+        return hipSuccess;
+    };
+    // inlet-density bookkeeping after iteration iter
+    auto after_iter = [&](int iter) {
+        const bool window = is_window(iter);
+        h->prev_inlet_valid = window ? 1 : 0;
+        h->prev_inlet_buf = iter & 1;
+        h->prev_inlet_c = (double)(float)(iter - opts.statsbegin); // real() is REAL(4) (:358)
+        if (sample_step(iter)) h->nsamples++;
+    };
+    auto record_iter = [&](int iter) {
+        if (!is_window(iter)) return;
+        rec_iters.push_back(iter);
+        if ((int)rec_iters.size() == h->rec_cap) {
+            drain_records(h, rec_iters, opts.nprobe, opts.record_surface_p);
+            rec_iters.clear();
+        }
+    };
+    // a pair of iterations may share one two-step bulk pass when neither needs per-node
+    // work of the bulk (plot-step zeroing, u sampling, probes) and the records fit
+    auto pair_ok = [&](int iter) {
+        if (!h->tb_H || h->from_pops || h->chi || h->p.kernel == 1) return false;
+        for (int it = iter; it <= iter + 1; it++) {
+            if (plot_step(it) || sample_step(it)) return false;
+            if (is_window(it) && opts.nprobe > 0) return false;
+        }
+        const int nrec = (is_window(iter) ? 1 : 0) + (is_window(iter + 1) ? 1 : 0);
+        return (int)rec_iters.size() + nrec <= std::max(h->rec_cap, 0) || nrec == 0;
+    };
+    if (h->timing) HIPCHK(hipEventRecord(h->e0, h->st));
+    for (int s = 0; s < nsteps; s++) {
+        const int iter = iter_first + s;
+        if (s + 1 < nsteps && pair_ok(iter)) {
+            void *A = h->S[h->cur], *C = h->S[h->cur ^ 1];
+            const int slot1 = (int)rec_iters.size();
+            auto a1 = step_args(iter, A, h->S2, slot1);
+            auto bulk = a1;
+            bulk.dst = (S *)C;
+            bulk.nstrips = h->nstrips2;
+            bulk.ncg = h->ncg2;
+            bulk.ncb = h->ncb2;
+            HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
+            HIPCHK(launch_bulk2<S, T>(bulk, h->ncb2 * h->nstrips2, h->tb_H, h->st));
+            HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
+            HIPCHK(launch_edges<S, T>(a1, h->st2));
+            HIPCHK(side_wall(a1));
+            a1.nrect = h->nrect1;
+            std::memcpy(a1.rect, h->rect1, sizeof h->rect1);
+            HIPCHK(launch_band<S, T>(a1, h->st2));
+            after_iter(iter);
+            auto a2 = step_args(iter + 1, h->S2, C, slot1 + (is_window(iter) ? 1 : 0));
+            HIPCHK(launch_edges<S, T>(a2, h->st2));
+            HIPCHK(side_wall(a2));
+            a2.nrect = h->nrect2;
+            std::memcpy(a2.rect, h->rect2, sizeof h->rect2);
+            HIPCHK(launch_band<S, T>(a2, h->st2));
+            after_iter(iter + 1);
+            HIPCHK(hipEventRecord(h->evB, h->st2));
+            HIPCHK(hipEventRecord(h->evA, h->st));
+            h->cur ^= 1;
+            record_iter(iter);
+            record_iter(iter + 1);
+            s++;
+            continue;
+        }
+        auto a = step_args(iter, h->S[h->cur], h->S[h->cur ^ 1], (int)rec_iters.size());
+        if (h->from_pops) a.pops = (const T *)h->pops;
         const bool pops_step = h->from_pops;
         const int nb = (pops_step || a.simple || a.chi) ? simple_blocks : march_blocks;
         // bulk_n after wall_{n-1}; wall_n after bulk_{n-1}: both read state n-1 only
@@ -548,27 +681,13 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         HIPCHK(launch_bulk<S, T>(a, nb, h->st));
         HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
         HIPCHK(launch_edges<S, T>(a, h->st2));
-        if (h->has_b) HIPCHK(launch_boundary<S, T>(a, h->st2));
-        else if (h->prev_inlet_valid) {
-            auto f = a;
-            f.inlet_prev = h->d_inlet[h->prev_inlet_buf];
-            HIPCHK(launch_inlet_fold<S, T>(f, h->st2));
-        }
+        HIPCHK(side_wall(a));
         HIPCHK(hipEventRecord(h->evB, h->st2));
         HIPCHK(hipEventRecord(h->evA, h->st));
-        h->prev_inlet_valid = window ? 1 : 0;
-        h->prev_inlet_buf = iter & 1;
-        h->prev_inlet_c = (double)(float)(iter - opts.statsbegin); // real() is REAL(4) (:358)
+        after_iter(iter);
         h->cur ^= 1;
         h->from_pops = false;
-        if (a.sample) h->nsamples++;
-        if (window) {
-            rec_iters.push_back(iter);
-            if ((int)rec_iters.size() == h->rec_cap) {
-                drain_records(h, rec_iters, opts.nprobe, opts.record_surface_p);
-                rec_iters.clear();
-            }
-        }
+        record_iter(iter);
     }
     HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0)); // join the wall stream
     if (h->prev_inlet_valid) { // fold the last window step's inlet density now

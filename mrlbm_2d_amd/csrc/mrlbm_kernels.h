@@ -54,6 +54,8 @@ template <class S, class T> struct StepArgs {
     double inlet_prev_c;       // its mean counter
     double *rho_inf_mean;
     double *facc;              // batched runs: running sums of Fd, Fl, Fl^2 and the step count
+    int nrect;                 // band kernel: node rectangles x0, y0, w, h
+    int rect[6][4];
     const T *chi;              // Brinkman solid fraction (per-node bulk kernel only), or null
     BoundaryDev b;
 };
@@ -75,6 +77,13 @@ template <class S, class T>
 hipError_t launch_ghost_from_pops(const StepArgs<S, T> &a, T *ghost, hipStream_t st);
 template <class S, class T>
 hipError_t launch_stencil_moments(const StepArgs<S, T> &a, hipStream_t st);
+// Two steps in one pass over the plain interior (nodes two or more away from the edges and
+// from the dilated cylinder box), src = state t, dst = state t+2; strips of H rows.
+template <class S, class T>
+hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st);
+// Plain node update over a.rect[0..nrect) (boundary nodes skipped): the band sub-steps.
+template <class S, class T>
+hipError_t launch_band(const StepArgs<S, T> &a, hipStream_t st);
 // Batched runs: args[r] per run in device memory; tiles[t] = (run, bulk block of that run).
 template <class S, class T>
 hipError_t launch_bulk_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int variant, int grid,

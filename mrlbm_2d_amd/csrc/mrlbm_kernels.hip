@@ -287,6 +287,159 @@ template <class S, class T, bool REC, int H> __device__ void bulk_march_a(const 
                                up_cur, mid_cur, hu_prev, hu_cur, hm_cur, row_cls(bm, sm, t));
 }
 
+// ---------------------------------------------------------------------------------
+// Two steps per pass (temporal blocking).  A wave owns 64 aligned columns x0..x0+63 and
+// marches up a strip; per row it first advances t -> t+1 for its own columns and for one
+// halo column per side (x0-1 in lane 0, x0+64 in lane 63), then t+1 -> t+2 for its own
+// columns with the single-step aligned row (march_row_a).  State t is loaded with one
+// extra per-lane load per plane: lanes 0, 1, 62, 63 fetch x0-2, x0-1, x0+64, x0+65 (the
+// other lanes repeat their own column, which coalesces).  Every node update is the
+// single-step arithmetic, so two passes of this kernel give the bits of four steps.
+
+// Level 1 of row r: pulls from rows r-1 (carried "up" sets), r ("mid" sets) and r+1 (F9,
+// H9: the freshly rebuilt own and halo populations), for the own node (f) and the halo
+// node (g); returns their post-collision moments and rolls the carries.
+template <class S, class T>
+__device__ __forceinline__ void level1_row(const StepArgs<S, T> &a, bool left, bool right, int nbl, const T F9[Q],
+                                           const T H9[Q], T up_prev[3], T up_cur[3], T mid_cur[3], T hup_prev[3],
+                                           T hup_cur[3], T hmid_cur[3], T m1[6], T m1h[6])
+{
+    using F = typename FormOf<T>::F;
+    T f[Q], g[Q];
+    // Every cross-lane read is done by all 64 lanes before any lane selects: a shuffle
+    // inside a branch would read the registers of lanes that branch left inactive.
+    const T s1 = shup(mid_cur[1]), s3 = shdn(mid_cur[2]), t1 = shdn(hmid_cur[1]), t3 = shup(hmid_cur[2]);
+    const T s5 = shup(up_prev[1]), s6 = shdn(up_prev[2]), t5 = shdn(hup_prev[1]), t6 = shup(hup_prev[2]);
+    const T s8 = shup(F9[8]), s7 = shdn(F9[7]), t8 = shdn(H9[8]), t7 = shup(H9[7]);
+    const T z0 = __shfl(hmid_cur[0], nbl, 64), z2 = __shfl(hup_prev[0], nbl, 64), z4 = __shfl(H9[4], nbl, 64);
+    // own node: x-shifted populations from the neighbour lanes, or for lanes 0 / 63 from
+    // the halo populations of lanes 1 / 62
+    f[0] = mid_cur[0];
+    f[1] = left ? t1 : s1;
+    f[3] = right ? t3 : s3;
+    f[2] = up_prev[0];
+    f[5] = left ? t5 : s5;
+    f[6] = right ? t6 : s6;
+    f[4] = F9[4];
+    f[8] = left ? t8 : s8;
+    f[7] = right ? t7 : s7;
+    // halo node (x0-1 in lane 0, x0+64 in lane 63): its own column is the halo of lane
+    // 1 / 62; the column beyond is this lane's halo, the column inside this lane's own
+    g[0] = z0;
+    g[2] = z2;
+    g[4] = z4;
+    g[1] = left ? hmid_cur[1] : mid_cur[1];
+    g[5] = left ? hup_prev[1] : up_prev[1];
+    g[8] = left ? H9[8] : F9[8];
+    g[3] = left ? mid_cur[2] : hmid_cur[2];
+    g[6] = left ? up_prev[2] : hup_prev[2];
+    g[7] = left ? F9[7] : H9[7];
+    F::moments(f, m1);
+    collide<T>(m1, a.om, a.om1, a.incomp_bulk);
+    F::moments(g, m1h);
+    collide<T>(m1h, a.om, a.om1, a.incomp_bulk);
+    up_prev[0] = up_cur[0]; up_prev[1] = up_cur[1]; up_prev[2] = up_cur[2];
+    up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
+    mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
+    hup_prev[0] = hup_cur[0]; hup_prev[1] = hup_cur[1]; hup_prev[2] = hup_cur[2];
+    hup_cur[0] = H9[2]; hup_cur[1] = H9[5]; hup_cur[2] = H9[6];
+    hmid_cur[0] = H9[0]; hmid_cur[1] = H9[1]; hmid_cur[2] = H9[3];
+}
+
+// Output nodes of the two-step pass: two or more nodes from the outer ring and outside the
+// cylinder box dilated by one (everything else is finished by the band sub-steps).
+template <class S, class T> __device__ __forceinline__ bool two_step_node(const StepArgs<S, T> &a, int c, int j)
+{
+    if (c < 2 || c > a.nx - 3 || j < 2 || j > a.ny - 3) return false;
+    return !(a.bw > 0 && c >= a.bx0 - 1 && c <= a.bx0 + a.bw && j >= a.by0 - 1 && j <= a.by0 + a.bh);
+}
+
+template <class S, class T, int H> __device__ void bulk_march2(const StepArgs<S, T> &a, int bb)
+{
+    using F = typename FormOf<T>::F;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int2 tile = bulk_tile(a, bb);
+    const int cbi = tile.x, st = tile.y;
+    const int g = cbi * 4 + wave;
+    if (g >= a.ncg) return;
+    const int x0 = g * 64, c = x0 + lane; // c < pitch: the pitch is a multiple of 64
+    const bool left = lane == 0, right = lane == 63;
+    const int hx = lane == 0 ? x0 - 2 : (lane == 1 ? x0 - 1 : (lane == 62 ? x0 + 64 : (lane == 63 ? x0 + 65 : c)));
+    const int hc = min(max(hx, 0), a.pitch - 1);
+    const int nbl = left ? 1 : (right ? 62 : lane);
+    const int j0 = 2 + st * H; // output rows j0 .. j0+H-1 at t+2
+    S m[H + 4][6], h[H + 4][6];
+#pragma unroll
+    for (int t = 0; t < H + 4; t++) { // state t rows j0-2 .. j0+H+1
+        const int r = min(j0 - 2 + t, a.ny - 1);
+        load_moments(a, c, r, m[t]);
+        load_moments(a, hc, r, h[t]);
+    }
+    T F9[Q], H9[Q];
+    T up_prev[3], up_cur[3], mid_cur[3], hup_prev[3], hup_cur[3], hmid_cur[3];
+    F::recon(m[0], F9);
+    F::recon(h[0], H9);
+    up_prev[0] = F9[2]; up_prev[1] = F9[5]; up_prev[2] = F9[6];
+    hup_prev[0] = H9[2]; hup_prev[1] = H9[5]; hup_prev[2] = H9[6];
+    F::recon(m[1], F9);
+    F::recon(h[1], H9);
+    up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
+    mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
+    hup_cur[0] = H9[2]; hup_cur[1] = H9[5]; hup_cur[2] = H9[6];
+    hmid_cur[0] = H9[0]; hmid_cur[1] = H9[1]; hmid_cur[2] = H9[3];
+    // level 1, rows j0-1 and j0: the first two t+1 rows seed the level-2 march
+    T m1[6], m1h[6];
+    T G9[Q], K9[Q], vup_prev[3], vup_cur[3], vmid_cur[3], ku_prev[1], ku_cur[1], km_cur[1];
+    F::recon(m[2], F9);
+    F::recon(h[2], H9);
+    level1_row(a, left, right, nbl, F9, H9, up_prev, up_cur, mid_cur, hup_prev, hup_cur, hmid_cur, m1, m1h);
+    F::recon(m1, G9);
+    F::recon(m1h, K9);
+    vup_prev[0] = G9[2]; vup_prev[1] = G9[5]; vup_prev[2] = G9[6];
+    ku_prev[0] = left ? K9[5] : K9[6];
+    F::recon(m[3], F9);
+    F::recon(h[3], H9);
+    level1_row(a, left, right, nbl, F9, H9, up_prev, up_cur, mid_cur, hup_prev, hup_cur, hmid_cur, m1, m1h);
+    F::recon(m1, G9);
+    F::recon(m1h, K9);
+    vup_cur[0] = G9[2]; vup_cur[1] = G9[5]; vup_cur[2] = G9[6];
+    vmid_cur[0] = G9[0]; vmid_cur[1] = G9[1]; vmid_cur[2] = G9[3];
+    ku_cur[0] = left ? K9[5] : K9[6];
+    km_cur[0] = left ? K9[1] : K9[3];
+#pragma unroll
+    for (int t = 0; t < H; t++) {
+        // level 1 of row j0+1+t, then level 2 of row j0+t
+        F::recon(m[t + 4], F9);
+        F::recon(h[t + 4], H9);
+        level1_row(a, left, right, nbl, F9, H9, up_prev, up_cur, mid_cur, hup_prev, hup_cur, hmid_cur, m1, m1h);
+        march_row_a<S, T, false>(a, c, two_step_node(a, c, j0 + t), j0 + t, m1, m1h, left, right, vup_prev, vup_cur,
+                                 vmid_cur, ku_prev, ku_cur, km_cur, 0);
+    }
+}
+
+template <class S, class T, int H> __global__ void __launch_bounds__(256) bulk2_kernel(StepArgs<S, T> a)
+{
+    bulk_march2<S, T, H>(a, blockIdx.x);
+}
+
+// Plain node update over the band rectangles of a two-step pair (boundary nodes are
+// skipped by node_update; the ring is never inside a rectangle).
+template <class S, class T> __global__ void __launch_bounds__(256) band_kernel(StepArgs<S, T> a)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int q = 0; q < a.nrect; q++) {
+        const long long n = (long long)a.rect[q][2] * a.rect[q][3];
+        if (t < n) {
+            const int i = a.rect[q][0] + (int)(t % a.rect[q][2]), j = a.rect[q][1] + (int)(t / a.rect[q][2]);
+            T f[Q];
+            gather_generic(a, i, j, f);
+            node_update(a, i, j, f, false);
+            return;
+        }
+        t -= n;
+    }
+}
+
 // Bulk, one node per thread with the generic gather: the first step after
 // mrlbm_set_populations (populations in memory) and a checking path.
 template <class S, class T> __device__ void bulk_simple(const StepArgs<S, T> &a, int bb)
@@ -678,6 +831,24 @@ hipError_t launch_side_batch(const StepArgs<S, T> *args, int nruns, int max_ring
     }
     return hipGetLastError();
 }
+template <class S, class T> hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st)
+{
+    const dim3 g(nblocks), b(256);
+    switch (H) {
+    case 2: hipLaunchKernelGGL((bulk2_kernel<S, T, 2>), g, b, 0, st, a); break;
+    case 4: hipLaunchKernelGGL((bulk2_kernel<S, T, 4>), g, b, 0, st, a); break;
+    case 8: hipLaunchKernelGGL((bulk2_kernel<S, T, 8>), g, b, 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template <class S, class T> hipError_t launch_band(const StepArgs<S, T> &a, hipStream_t st)
+{
+    long long n = 0;
+    for (int q = 0; q < a.nrect; q++) n += (long long)a.rect[q][2] * a.rect[q][3];
+    if (n > 0) hipLaunchKernelGGL((band_kernel<S, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
 template <class S, class T> hipError_t launch_stencil_moments(const StepArgs<S, T> &a, hipStream_t st)
 {
     hipLaunchKernelGGL((stencil_moments_kernel<S, T>), dim3((a.b.nU + 255) / 256), dim3(256), 0, st, a);
@@ -712,6 +883,8 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_boundary<S, T>(const StepArgs<S, T> &, hipStream_t);              \
     template hipError_t launch_edges<S, T>(const StepArgs<S, T> &, hipStream_t);                 \
     template hipError_t launch_stencil_moments<S, T>(const StepArgs<S, T> &, hipStream_t);       \
+    template hipError_t launch_bulk2<S, T>(const StepArgs<S, T> &, int, int, hipStream_t);       \
+    template hipError_t launch_band<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
     template hipError_t launch_bulk_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, int, hipStream_t); \
     template hipError_t launch_side_batch<S, T>(const StepArgs<S, T> *, int, int, int, hipStream_t); \
     template int bulk_batch_resident_blocks<S, T>(int);                                          \

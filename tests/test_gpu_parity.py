@@ -15,16 +15,17 @@ from tests import golden_cases as gc
 pytestmark = pytest.mark.gpu
 
 
-def run_golden(name, kernel=0):
+def run_golden(name, kernel=0, time_blocking=0):
     g = gc.load(name)
     p = g["params"]
     par = params_from_namelist(p)
     par.kernel = kernel
+    par.time_blocking = time_blocking
     lat = Lattice(par)
     if "f0" in g:
         lat.set_populations(g["f0"])
     n = int(g["nsteps"])
-    print(f"[gpu] golden {name} {lat.nx}x{lat.ny} steps={n} kernel={kernel}", flush=True)
+    print(f"[gpu] golden {name} {lat.nx}x{lat.ny} steps={n} kernel={kernel} time_blocking={time_blocking}", flush=True)
     opts = step_opts_from_namelist(p)
     # stop at iter n, where the reference last appended to rho_mean_inlet.dat (every 100 steps)
     lat.step(2, n - 1, opts)
@@ -33,9 +34,12 @@ def run_golden(name, kernel=0):
     return g, lat
 
 
+@pytest.mark.parametrize("time_blocking", [0, 4])
 @pytest.mark.parametrize("name", gc.names())
-def test_fp64_matches_reference(name):
-    g, lat = run_golden(name)
+def test_fp64_matches_reference(name, time_blocking):
+    """Also with two steps per bulk pass: pairs of steps share one pass wherever no
+    plot-step zeroing or probe falls inside them, single steps elsewhere."""
+    g, lat = run_golden(name, time_blocking=time_blocking)
     with lat:
         rho, ux, uy = lat.fields()
         np.testing.assert_array_equal(rho, g["rho"])
@@ -73,6 +77,17 @@ def test_bulk_kernel_variants_agree(name, kernel):
         np.testing.assert_array_equal(uy, g["uy"])
 
 
+@pytest.mark.parametrize("time_blocking", [2, 8])
+@pytest.mark.parametrize("name", ["p1_walls", "s_periodic", "s_xyper_noise", "chan"])
+def test_time_blocking_strips_agree(name, time_blocking):
+    g, lat = run_golden(name, time_blocking=time_blocking)
+    with lat:
+        rho, ux, uy = lat.fields()
+        np.testing.assert_array_equal(rho, g["rho"])
+        np.testing.assert_array_equal(ux, g["ux"])
+        np.testing.assert_array_equal(uy, g["uy"])
+
+
 @pytest.mark.parametrize("name", ["p1_walls", "s_xyper_noise", "s_noise_re60"])
 def test_fp32_variants_identical_and_close_to_reference(name):
     """FP32 deviation form: every bulk variant computes the same per-node arithmetic, so
@@ -81,9 +96,10 @@ def test_fp32_variants_identical_and_close_to_reference(name):
     g = gc.load(name)
     p = g["params"]
     results = {}
-    for kernel in (0, 1, 2, 4, 8, 16, 304, 308, 316):
+    for kernel in (0, 1, 2, 4, 8, 16, 304, 308, 316, 1002, 1004, 1008):
         par = params_from_namelist(p, precision=capi.MRLBM_FP32_DEV)
-        par.kernel = kernel
+        par.kernel = kernel if kernel < 1000 else 0
+        par.time_blocking = kernel - 1000 if kernel >= 1000 else 0
         with Lattice(par) as lat:
             if "f0" in g:
                 lat.set_populations(g["f0"])
