@@ -200,6 +200,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (for profiler counter scaling)")
@@ -219,8 +220,10 @@ def main():
     print(f"[bench] FP64 plain form {NX}x{NY}", file=sys.stderr, flush=True)
     r64 = time_form(capi.MRLBM_FP64, args.steps, args.warmup, strip_rows=args.strip_rows)
     print(f"[bench] FP64: {r64}", file=sys.stderr, flush=True)
-    print("[bench] drag adjoint, 512x128", file=sys.stderr, flush=True)
-    adj = adjoint_bench()
+    adj = None
+    if not args.no_adjoint:
+        print("[bench] drag adjoint, 512x128", file=sys.stderr, flush=True)
+        adj = adjoint_bench()
     batch = None
     if not args.no_batch:
         print("[bench] batched runs", file=sys.stderr, flush=True)

@@ -308,7 +308,9 @@ extern "C" int mrlbm_batch_create(const mrlbm_params *runs, int32_t nruns, const
     for (int r = 0; r < nruns; r++) {
         if ((runs[r].precision == MRLBM_FP32_DEV) != b->f32) return fail(MRLBM_E_ARG, "every run needs the same precision");
         mrlbm_handle *h = nullptr;
-        const int rc = mrlbm_create(&runs[r], &h);
+        mrlbm_params pr = runs[r];
+        pr.time_blocking = 0; // the batch steps its runs with its own launches
+        const int rc = mrlbm_create(&pr, &h);
         if (rc) return fail(rc, "run " + std::to_string(r) + ": " + mrlbm_last_error(nullptr));
         b->runs.push_back(h);
         b->nodes += (int64_t)h->nx * h->ny;

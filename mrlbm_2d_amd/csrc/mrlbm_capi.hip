@@ -599,9 +599,10 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         a.inlet_prev_c = h->prev_inlet_c;
         return a;
     };
-    // curved wall (or, without a cylinder, the inlet-density fold) of one iteration
+    // curved wall (or, without a cylinder, the inlet-density fold) of one iteration, after
+    // launch_side has done the ring, the bands and the wall's stencil nodes
     auto side_wall = [&](const StepArgs<S, T> &a) -> hipError_t {
-        if (h->has_b) return launch_boundary<S, T>(a, h->st2);
+        if (h->has_b) return launch_wall<S, T>(a, h->st2);
         if (h->prev_inlet_valid) {
             auto f = a;
             f.inlet_prev = h->d_inlet[h->prev_inlet_buf];
@@ -651,18 +652,16 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
             HIPCHK(launch_bulk2<S, T>(bulk, h->ncb2 * h->nstrips2, h->tb_H, h->st));
             HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
-            HIPCHK(launch_edges<S, T>(a1, h->st2));
-            HIPCHK(side_wall(a1));
             a1.nrect = h->nrect1;
             std::memcpy(a1.rect, h->rect1, sizeof h->rect1);
-            HIPCHK(launch_band<S, T>(a1, h->st2));
+            HIPCHK(launch_side<S, T>(a1, h->st2));
+            HIPCHK(side_wall(a1));
             after_iter(iter);
             auto a2 = step_args(iter + 1, h->S2, C, slot1 + (is_window(iter) ? 1 : 0));
-            HIPCHK(launch_edges<S, T>(a2, h->st2));
-            HIPCHK(side_wall(a2));
             a2.nrect = h->nrect2;
             std::memcpy(a2.rect, h->rect2, sizeof h->rect2);
-            HIPCHK(launch_band<S, T>(a2, h->st2));
+            HIPCHK(launch_side<S, T>(a2, h->st2));
+            HIPCHK(side_wall(a2));
             after_iter(iter + 1);
             HIPCHK(hipEventRecord(h->evB, h->st2));
             HIPCHK(hipEventRecord(h->evA, h->st));
@@ -680,7 +679,7 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
         HIPCHK(launch_bulk<S, T>(a, nb, h->st));
         HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
-        HIPCHK(launch_edges<S, T>(a, h->st2));
+        HIPCHK(launch_side<S, T>(a, h->st2));
         HIPCHK(side_wall(a));
         HIPCHK(hipEventRecord(h->evB, h->st2));
         HIPCHK(hipEventRecord(h->evA, h->st));

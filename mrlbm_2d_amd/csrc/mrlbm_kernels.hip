@@ -440,6 +440,36 @@ template <class S, class T> __global__ void __launch_bounds__(256) band_kernel(S
     }
 }
 
+// The side stream's node work of one step in one launch: the outer ring (edge update),
+// the band rectangles of a two-step pair (plain update) and the curved wall's stencil
+// nodes (pre-collision moments for boundary_kernel).  All three read only the source
+// state and write disjoint nodes.
+template <class S, class T> __global__ void __launch_bounds__(256) side_kernel(StepArgs<S, T> a, int nring, int nband)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nring) {
+        edge_body(a, (int)t);
+        return;
+    }
+    t -= nring;
+    if (t < nband) {
+        for (int q = 0; q < a.nrect; q++) {
+            const long long n = (long long)a.rect[q][2] * a.rect[q][3];
+            if (t < n) {
+                const int i = a.rect[q][0] + (int)(t % a.rect[q][2]), j = a.rect[q][1] + (int)(t / a.rect[q][2]);
+                T f[Q];
+                gather_generic(a, i, j, f);
+                node_update(a, i, j, f, false);
+                return;
+            }
+            t -= n;
+        }
+        return;
+    }
+    t -= nband;
+    if (a.has_boundary) stencil_moments_body(a, (int)t);
+}
+
 // Bulk, one node per thread with the generic gather: the first step after
 // mrlbm_set_populations (populations in memory) and a checking path.
 template <class S, class T> __device__ void bulk_simple(const StepArgs<S, T> &a, int bb)
@@ -849,6 +879,20 @@ template <class S, class T> hipError_t launch_band(const StepArgs<S, T> &a, hipS
     if (n > 0) hipLaunchKernelGGL((band_kernel<S, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
+template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st)
+{
+    const int nring = 2 * a.nx + 2 * (a.ny - 2);
+    long long nband = 0;
+    for (int q = 0; q < a.nrect; q++) nband += (long long)a.rect[q][2] * a.rect[q][3];
+    const long long n = nring + nband + (a.has_boundary ? a.b.nU : 0);
+    hipLaunchKernelGGL((side_kernel<S, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, nring, (int)nband);
+    return hipGetLastError();
+}
+template <class S, class T> hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st)
+{
+    hipLaunchKernelGGL((boundary_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
 template <class S, class T> hipError_t launch_stencil_moments(const StepArgs<S, T> &a, hipStream_t st)
 {
     hipLaunchKernelGGL((stencil_moments_kernel<S, T>), dim3((a.b.nU + 255) / 256), dim3(256), 0, st, a);
@@ -885,6 +929,8 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_stencil_moments<S, T>(const StepArgs<S, T> &, hipStream_t);       \
     template hipError_t launch_bulk2<S, T>(const StepArgs<S, T> &, int, int, hipStream_t);       \
     template hipError_t launch_band<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
+    template hipError_t launch_side<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
+    template hipError_t launch_wall<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
     template hipError_t launch_bulk_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, int, hipStream_t); \
     template hipError_t launch_side_batch<S, T>(const StepArgs<S, T> *, int, int, int, hipStream_t); \
     template int bulk_batch_resident_blocks<S, T>(int);                                          \
