@@ -3,11 +3,15 @@
 Workload (the perf configuration of the metric): 8192 x 2048 channel, cylinder D = 128
 centred at (1024, 1025), parabolic inlet u_max = 0.05, Re = 200, seeded +-1e-4 initial
 velocity noise.  Both precision forms run: FP32 deviation storage (the headline `value`)
-and plain FP64.  A "step" is one lattice update of every node (one launch).
+and plain FP64.  A "step" is one lattice update of every node; the default path advances
+two steps per launch of the bulk kernel (temporal blocking), bit-identical to single steps.
 
-Prints ONE JSON line.  `roofline` prices the step kernel against MI355X's 8 TB/s HBM:
-achieved = algorithmic bytes per launch (12 moment words per node: 6 read + 6 written)
-/ mean kernel time measured with HIP events on the library's stream.
+Prints ONE JSON line.  `roofline` prices the dominant kernel against MI355X's 8 TB/s HBM:
+achieved = algorithmic bytes per launch (12 moment words per node and step: 6 read + 6
+written, two steps) / mean launch time measured with HIP events on the library's stream;
+traffic = the launch's measured HBM bytes (PMC), below the algorithmic bytes because the
+intermediate step never leaves the chip.  Also: the 512x128 drag adjoint (adjoint / forward
+time) and 256 batched runs (aggregate MLUPS).
 `cpu_baseline` times the oracle (the CPU restatement of the reference step) on a bounded
 number of steps of the same grid.  The reference program itself is not timed here: its
 set-up writes every fluid node to formatted text files, which at this grid size runs for
@@ -27,10 +31,12 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per bulk launch of the default variants at 8192 x 2048, from rocprofv3 FETCH_SIZE
-# and WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1);
-# the summary and the commands are in profiles/bulk_kernel_8192x2048_hbm_bytes.md
-TRAFFIC_BYTES = {"f32": 906_824_379, "f64": 1_923_698_699}
+# HBM bytes per launch of the dominant kernel at 8192 x 2048 (the two-step bulk pass, two
+# steps per launch: FP32 strips of 4 rows, FP64 of 2), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1);
+# the summary and the commands are in profiles/bulk2_kernel_8192x2048_hbm_bytes.md
+TRAFFIC_BYTES = {"f32": 901_869_006, "f64": 1_706_013_162}
+STEPS_PER_LAUNCH = 2
 NX, NY = 8192, 2048
 XC, YC, RC = 1024.0, 1025.0, 64.0
 UMAX, RE = 0.05, 200.0
@@ -71,10 +77,23 @@ def time_form(precision, steps, warmup, **kw):
     lat.close()
     n = info.nx * info.ny
     mlups = n * steps / (wall * 1e6)
-    bytes_per_launch = int(info.bytes_per_step)
-    achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    # kern_ms: mean time per step between HIP events on the bulk kernel's stream over the
+    # timed steps (the side stream is joined into it); one launch covers two steps
+    bytes_per_step = int(info.bytes_per_step)
+    achieved = bytes_per_step / (kern_ms * 1e-3) / 1e9
     return dict(mlups=mlups, wall_s=wall, ms_per_step=1e3 * wall / steps, kernel_ms=kern_ms,
-                bytes_per_launch=bytes_per_launch, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS)
+                bytes_per_step=bytes_per_step, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS)
+
+
+def roofline(r, kernel, traffic):
+    """Roofline of the dominant kernel: algorithmic bytes of one launch (two steps of 12 moment
+    words per node) over its mean launch time, against 8 TB/s; traffic = measured HBM bytes
+    of one launch (PMC)."""
+    return {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(r["frac"], 4), "traffic": traffic, "kernel": kernel,
+            "steps_per_launch": STEPS_PER_LAUNCH,
+            "algorithmic_bytes_per_launch": STEPS_PER_LAUNCH * r["bytes_per_step"],
+            "launch_ms": round(STEPS_PER_LAUNCH * r["kernel_ms"], 5)}
 
 
 def cpu_baseline(nsteps=2):
@@ -249,13 +268,10 @@ def main():
         "data": "synthetic (seeded +-1e-4 velocity noise)",
         "config": {"workload": "8192x2048 channel, cylinder D=128 at (1024,1025), "
                                "parabolic inlet u_max=0.05, Re=200", "nx": NX, "ny": NY},
-        "roofline": {"bound": "hbm", "achieved": round(r32["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(r32["frac"], 4),
-                     "traffic": TRAFFIC_BYTES["f32"] if default_shape else None},
+        "roofline": roofline(r32, "bulk2_kernel<float,float,4>", TRAFFIC_BYTES["f32"] if default_shape else None),
         "fp64": {"value": round(r64["mlups"], 1), "ms_per_step": round(r64["ms_per_step"], 5),
-                 "roofline": {"bound": "hbm", "achieved": round(r64["achieved_gbs"], 1),
-                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(r64["frac"], 4),
-                              "traffic": TRAFFIC_BYTES["f64"] if default_shape else None}},
+                 "roofline": roofline(r64, "bulk2_kernel<double,double,2>",
+                                      TRAFFIC_BYTES["f64"] if default_shape else None)},
         "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3),
         "adjoint": adj,
         "batched": batch,
