@@ -11,10 +11,20 @@ reference's separate program 2d_statistics.f90 (statistics_calculation).
   (:145-165).
 
 Raises StatisticsError where the reference prints "Could not detect two peaks." and stops.
+
+Drop-in for the reference's post-processing program, run in a run directory:
+
+    python -m mrlbm_2d_amd.statistics [run_dir]
+
+reads input_stat.dat, data_mean/forces_time.dat, theta.dat and the unformatted
+p_surface{,_smooth}_time.dat, and writes plots/cp.dat, cp_fit.dat, cd_time.dat,
+cl_time.dat and drag_lift.dat with the same rows (2d_statistics.f90:29-178).
 """
 from __future__ import annotations
 
 import math
+import os
+import sys
 from dataclasses import dataclass
 
 import numpy as np
@@ -92,3 +102,68 @@ def cp_profile(theta, cp):
     theta = np.asarray(theta, dtype=np.float64)
     keep = theta <= math.pi
     return np.stack([np.abs((theta[keep] * 180.0 / math.pi) - 180.0), np.asarray(cp)[keep]], axis=1)
+
+
+def read_unformatted_series(path, nrec):
+    """`nrec` records of a Fortran sequential unformatted file whose records are
+    (int32 iteration, float64 values...) between 4-byte length markers (:100-111)."""
+    raw = open(path, "rb").read()
+    iters, rows, pos = [], [], 0
+    for _ in range(nrec):
+        m = int(np.frombuffer(raw[pos:pos + 4], np.int32)[0])
+        rec = raw[pos + 4:pos + 4 + m]
+        iters.append(int(np.frombuffer(rec[:4], np.int32)[0]))
+        rows.append(np.frombuffer(rec[4:], np.float64).copy())
+        pos += 8 + m
+    return np.array(iters, np.int32), np.array(rows)
+
+
+def _fmt(v):
+    return f" {v!r}" if isinstance(v, float) else f" {v}"
+
+
+def run(directory="."):
+    """The reference post-processor on the files of one run directory."""
+    from .namelist import read_input_stat
+    prm = read_input_stat(os.path.join(directory, "input_stat.dat"))
+    nbct, r_cy, uo, rho_infty = int(prm["nbct"]), float(prm["r_cy"]), float(prm["uo"]), float(prm["rho_infty"])
+    dm = os.path.join(directory, "data_mean")
+    rows = [l.split() for l in open(os.path.join(dm, "forces_time.dat")) if l.strip()]
+    nstep = len(rows)
+    fd = np.array([float(r[1]) for r in rows])
+    fl = np.array([float(r[2]) for r in rows])
+    theta = np.array([float(l.split()[0]) for l in open(os.path.join(dm, "theta.dat")) if l.strip()][:nbct])
+    _, ps = read_unformatted_series(os.path.join(dm, "p_surface_time.dat"), nstep)
+    _, pf = read_unformatted_series(os.path.join(dm, "p_surface_smooth_time.dat"), nstep)
+    r = compute(fd, fl, uo, rho_infty, r_cy, ps=ps[:, :nbct], ps_fit=pf[:, :nbct])
+    f_norm = ((0.50 * rho_infty) * (uo * uo)) * (2.0 * r_cy)
+    out = os.path.join(directory, "plots")
+    os.makedirs(out, exist_ok=True)
+    for name, cp in (("cp.dat", r.cp), ("cp_fit.dat", r.cp_fit)):
+        with open(os.path.join(out, name), "w") as fh:
+            for a, b in cp_profile(theta, cp):
+                fh.write(_fmt(float(a)) + _fmt(float(b)) + "\n")
+    ce = r.cycle_start + r.n_cycle * r.period
+    for name, sig in (("cd_time.dat", fd), ("cl_time.dat", fl)):
+        with open(os.path.join(out, name), "w") as fh:
+            for i in range(r.cycle_start, ce + 1):
+                # real(i - cycle_start) / real(period) is a REAL(4) quotient
+                x = float(np.float32(i - r.cycle_start) / np.float32(r.period))
+                fh.write(_fmt(x) + _fmt(float(sig[i - 1] / f_norm)) + "\n")
+    lines = [("Strouhal number (St)  ", float(r.st)), ("Drag coefficinet (C_D)  ", float(r.cd)),
+             ("Lift coefficinet (C_L ", float(r.cl_rms))]
+    with open(os.path.join(out, "drag_lift.dat"), "w") as fh:
+        for k, v in lines:
+            fh.write(f" {k}:{v!r}\n")
+    print(f"Averging over {r.n_cycle} cycle(s)")
+    for k, v in lines:
+        print(f" {k}:{v!r}")
+    return r
+
+
+if __name__ == "__main__":
+    try:
+        run(sys.argv[1] if len(sys.argv) > 1 else ".")
+    except StatisticsError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

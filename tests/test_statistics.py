@@ -1,5 +1,7 @@
 """The product-side post-processor (mrlbm_2d_amd.statistics) against the reference's
 2d_statistics.f90 run on a synthetic shedding series (tests/golden/stats_synth.npz)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -43,3 +45,37 @@ def test_sinusoid_closed_form():
 def test_two_peaks_required():
     with pytest.raises(S.StatisticsError):
         S.compute(np.ones(10), np.arange(10.0), 0.1, 1.0, 1.0)
+
+
+@pytest.mark.parametrize("name", gc.names("statistics"))
+def test_drop_in_program_reads_and_writes_the_reference_files(name, tmp_path):
+    """The run-directory entry point on the reference's file formats: the same inputs the
+    reference program read for the fixture give its St, C_D, C_L and Cp rows."""
+    from tests.golden.make_fixtures import write_unformatted
+    g = gc.load(name)
+    dm = tmp_path / "data_mean"
+    dm.mkdir()
+    nbct = len(g["theta"])
+    (tmp_path / "input_stat.dat").write_text(
+        f"&parameters\n\tnbct = {nbct}\n\tr_cy = {float(g['r_cy'])!r}\n\tuo = {float(g['uo'])!r}\n"
+        f"\trho_infty = {float(g['rho_infty'])!r}\n/\n")
+    with open(dm / "forces_time.dat", "w") as fh:
+        for it, a, b in zip(g["iters"], g["fd"], g["fl"]):
+            fh.write(f" {int(it)} {float(a)!r} {float(b)!r}\n")
+    with open(dm / "theta.dat", "w") as fh:
+        for x in g["theta"]:
+            fh.write(f" {float(x)!r}\n")
+    write_unformatted(str(dm / "p_surface_time.dat"), g["iters"], g["ps"])
+    write_unformatted(str(dm / "p_surface_smooth_time.dat"), g["iters"], g["pfit"])
+    S.run(str(tmp_path))
+    vals = {}
+    for line in open(tmp_path / "plots" / "drag_lift.dat"):
+        key, _, v = line.rpartition(":")
+        vals[key.strip()] = float(v)
+    np.testing.assert_allclose(vals["Strouhal number (St)"], g["st"], rtol=1e-15)
+    np.testing.assert_allclose(vals["Drag coefficinet (C_D)"], g["cd"], rtol=1e-15)
+    np.testing.assert_allclose(vals["Lift coefficinet (C_L"], g["cl_rms"], rtol=1e-15)
+    np.testing.assert_allclose(np.loadtxt(tmp_path / "plots" / "cp.dat"), g["cp"], rtol=1e-14, atol=1e-14)
+    cd_t = np.loadtxt(tmp_path / "plots" / "cd_time.dat")
+    assert cd_t[0, 0] == 0.0 and np.all(np.diff(cd_t[:, 0]) > 0)
+    assert os.path.exists(tmp_path / "plots" / "cl_time.dat")
