@@ -211,14 +211,17 @@ typedef struct mrlbm_batch_opts {
                                      0 one block per tile, -1 autotune */
     int32_t graph_steps;          /* steps per captured graph (even; 0 = 50) */
     int32_t tune_steps;           /* timed steps per autotune candidate (0 = 20) */
-    int32_t reserved[4];
+    int32_t time_blocking;        /* -1 autotune, 0 one step per pass, 2/4/8 two steps per pass
+                                     with strips of that many rows (as mrlbm_params) */
+    int32_t reserved[3];
 } mrlbm_batch_opts;
 
 typedef struct mrlbm_batch_info {
     int32_t nruns, kernel, persistent, grid, ntiles, graph_steps;
     int64_t nodes;                /* sum of nx*ny over the runs */
+    int32_t time_blocking;        /* chosen two-step strip height, 0 = one step per pass */
     int32_t ntune;                /* autotune candidates timed */
-    int32_t tune_kernel[16], tune_persistent[16];
+    int32_t tune_kernel[16], tune_persistent[16], tune_tb[16];
     double tune_ms[16];           /* mean step time of each candidate */
 } mrlbm_batch_info;
 

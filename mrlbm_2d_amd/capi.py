@@ -87,15 +87,17 @@ class AdjointResult(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [
         ("kernel", C.c_int32), ("persistent", C.c_int32), ("graph_steps", C.c_int32),
-        ("tune_steps", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("tune_steps", C.c_int32), ("time_blocking", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 
 class BatchInfo(C.Structure):
     _fields_ = [
         ("nruns", C.c_int32), ("kernel", C.c_int32), ("persistent", C.c_int32), ("grid", C.c_int32),
-        ("ntiles", C.c_int32), ("graph_steps", C.c_int32), ("nodes", C.c_int64), ("ntune", C.c_int32),
-        ("tune_kernel", C.c_int32 * 16), ("tune_persistent", C.c_int32 * 16), ("tune_ms", C.c_double * 16),
+        ("ntiles", C.c_int32), ("graph_steps", C.c_int32), ("nodes", C.c_int64),
+        ("time_blocking", C.c_int32), ("ntune", C.c_int32),
+        ("tune_kernel", C.c_int32 * 16), ("tune_persistent", C.c_int32 * 16), ("tune_tb", C.c_int32 * 16),
+        ("tune_ms", C.c_double * 16),
     ]
 
 

@@ -6,8 +6,10 @@
 // is four launches for all runs: the bulk over the tile table on one stream and, on a
 // second stream, the edges, the stencil moments and the boundary workgroups (one per run),
 // joined as in the single-run step loop.  Runs of graph_steps steps are captured once per
-// (start parity, recording mode) and replayed.  The autotuner times the candidate bulk
-// variants and grid modes on a saved copy of the states and keeps the fastest.
+// (start parity, recording mode) and replayed.  With two steps per pass, a pair of steps is
+// the two-step bulk over its own tile table plus two band sub-steps on the second stream,
+// as in the single-run loop.  The autotuner times the candidate bulk variants, grid modes
+// and two-step strip heights on a saved copy of the states and keeps the fastest.
 #include "mrlbm_handle.h"
 #include "mrlbm_lattice.h"
 
@@ -26,15 +28,22 @@ struct mrlbm_batch {
     std::vector<mrlbm_handle *> runs;
     bool f32 = false;
     int kernel = 0, persistent = 1, grid = 0, ntiles = 0, graph_steps = 50, tune_steps = 20;
-    int max_ring = 0, max_nU = 0;
+    int tb = 0, ntiles2 = 0;      // two steps per bulk pass: strip height (0 off), tiles
+    bool tb_possible = true;      // every run has the two-step scratch state
+    int max_side[3] = {0, 0, 0};  // side-kernel threads per run: single step, sub-step 1, 2
+    bool any_wall = false;
     int64_t nodes = 0;
     hipStream_t st = nullptr, st2 = nullptr;
     hipEvent_t evA = nullptr, evB = nullptr, e0 = nullptr, e1 = nullptr;
-    void *d_args[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}}; // [parity][record]
-    int2 *d_tiles = nullptr;
+    void *d_args[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}}; // single step [parity][record]
+    void *d_b2[2] = {nullptr, nullptr};                            // two-step bulk [parity]
+    void *d_s1[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // sub-step 1 [parity][record]
+    void *d_s2[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // sub-step 2 [parity][record]
+    int2 *d_tiles = nullptr, *d_tiles2 = nullptr;
     double *d_facc = nullptr; // 4 per run
     hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    std::vector<int> tune_kernel, tune_persistent;
+    int exec_flip = 0;        // parity change of one graph replay
+    std::vector<int> tune_kernel, tune_persistent, tune_tb;
     std::vector<double> tune_ms;
     std::string err;
 };
@@ -60,43 +69,96 @@ static void drop_graphs(mrlbm_batch *b)
             }
 }
 
-// Argument blocks and tile table for the runs' current bulk shape.
+template <class S, class T> static int upload_args(mrlbm_batch *b, void *&dst, const std::vector<StepArgs<S, T>> &v)
+{
+    if (!dst) BHIP(hipMalloc(&dst, v.size() * sizeof(StepArgs<S, T>)));
+    BHIP(hipMemcpy(dst, v.data(), v.size() * sizeof(StepArgs<S, T>), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <class T> static int upload_tiles(mrlbm_batch *b, int2 *&dst, const std::vector<int2> &v)
+{
+    if (dst) (void)hipFree(dst);
+    dst = nullptr;
+    BHIP(hipMalloc(&dst, std::max<size_t>(v.size(), 1) * sizeof(int2)));
+    if (!v.empty()) BHIP(hipMemcpy(dst, v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Argument blocks and tile tables for the runs' current bulk shape and b->tb.
 template <class S, class T> static int build_tables(mrlbm_batch *b)
 {
     drop_graphs(b);
     const int R = (int)b->runs.size();
-    std::vector<StepArgs<S, T>> args[2][2];
-    std::vector<int2> tiles;
-    b->max_ring = 0;
-    b->max_nU = 0;
+    std::vector<StepArgs<S, T>> args[2][2], b2[2], s1[2][2], s2[2][2];
+    std::vector<int2> tiles, tiles2;
+    for (int k = 0; k < 3; k++) b->max_side[k] = 0;
+    b->any_wall = false;
     for (int r = 0; r < R; r++) {
         mrlbm_handle *h = b->runs[r];
+        const int ring = 2 * h->nx + 2 * (h->ny - 2), nU = h->has_b ? h->bt.nU : 0;
+        auto band = [](const int rc[5][4], int n) {
+            int t = 0;
+            for (int q = 0; q < n; q++) t += rc[q][2] * rc[q][3];
+            return t;
+        };
+        b->max_side[0] = std::max(b->max_side[0], ring + nU);
+        b->any_wall = b->any_wall || h->has_b;
         for (int par = 0; par < 2; par++)
             for (int rec = 0; rec < 2; rec++) {
                 auto a = base_args<S, T>(h);
-                a.src = (const S *)h->S[par];
-                a.dst = (S *)h->S[par ^ 1];
                 a.pops = nullptr;
                 a.rec_slot = rec ? 0 : -1;
                 a.facc = rec ? b->d_facc + 4 * r : nullptr;
-                args[par][rec].push_back(a);
+                auto one = a;
+                one.src = (const S *)h->S[par];
+                one.dst = (S *)h->S[par ^ 1];
+                args[par][rec].push_back(one);
+                if (b->tb) {
+                    auto x = a; // sub-step 1: state t -> scratch
+                    x.src = (const S *)h->S[par];
+                    x.dst = (S *)h->S2;
+                    x.nrect = h->nrect1;
+                    std::memcpy(x.rect, h->rect1, sizeof h->rect1);
+                    s1[par][rec].push_back(x);
+                    auto y = a; // sub-step 2: scratch -> state t+2
+                    y.src = (const S *)h->S2;
+                    y.dst = (S *)h->S[par ^ 1];
+                    y.nrect = h->nrect2;
+                    std::memcpy(y.rect, h->rect2, sizeof h->rect2);
+                    s2[par][rec].push_back(y);
+                    if (rec == 0) {
+                        auto z = one; // two-step bulk: state t -> state t+2
+                        z.nstrips = (h->ny - 4 + b->tb - 1) / b->tb;
+                        z.ncg = h->ncg2;
+                        z.ncb = h->ncb2;
+                        b2[par].push_back(z);
+                    }
+                }
             }
         const int nb = h->ncb * h->nstrips;
         for (int t = 0; t < nb; t++) tiles.push_back(make_int2(r, t));
-        b->max_ring = std::max(b->max_ring, 2 * h->nx + 2 * (h->ny - 2));
-        if (h->has_b) b->max_nU = std::max(b->max_nU, h->bt.nU);
-    }
-    for (int par = 0; par < 2; par++)
-        for (int rec = 0; rec < 2; rec++) {
-            if (!b->d_args[par][rec]) BHIP(hipMalloc(&b->d_args[par][rec], (size_t)R * sizeof(StepArgs<S, T>)));
-            BHIP(hipMemcpy(b->d_args[par][rec], args[par][rec].data(), (size_t)R * sizeof(StepArgs<S, T>),
-                           hipMemcpyHostToDevice));
+        if (b->tb) {
+            b->max_side[1] = std::max(b->max_side[1], ring + band(h->rect1, h->nrect1) + nU);
+            b->max_side[2] = std::max(b->max_side[2], ring + band(h->rect2, h->nrect2) + nU);
+            const int nb2 = h->ncb2 * ((h->ny - 4 + b->tb - 1) / b->tb);
+            for (int t = 0; t < nb2; t++) tiles2.push_back(make_int2(r, t));
         }
-    if (b->d_tiles) (void)hipFree(b->d_tiles);
-    b->d_tiles = nullptr;
-    BHIP(hipMalloc(&b->d_tiles, tiles.size() * sizeof(int2)));
-    BHIP(hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    int rc = 0;
+    for (int par = 0; par < 2 && !rc; par++) {
+        for (int rec = 0; rec < 2 && !rc; rec++) {
+            rc = upload_args<S, T>(b, b->d_args[par][rec], args[par][rec]);
+            if (!rc && b->tb) rc = upload_args<S, T>(b, b->d_s1[par][rec], s1[par][rec]);
+            if (!rc && b->tb) rc = upload_args<S, T>(b, b->d_s2[par][rec], s2[par][rec]);
+        }
+        if (!rc && b->tb) rc = upload_args<S, T>(b, b->d_b2[par], b2[par]);
+    }
+    if (!rc) rc = upload_tiles<T>(b, b->d_tiles, tiles);
+    if (!rc) rc = upload_tiles<T>(b, b->d_tiles2, tiles2);
+    if (rc) return rc;
     b->ntiles = (int)tiles.size();
+    b->ntiles2 = (int)tiles2.size();
     const int resident = bulk_batch_resident_blocks<S, T>(b->runs[0]->variant);
     b->grid = (b->persistent && resident > 0) ? std::min(resident, b->ntiles) : b->ntiles;
     return 0;
@@ -106,13 +168,47 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
 template <class S, class T> static int launch_step(mrlbm_batch *b, int par, int rec)
 {
     const auto *args = (const StepArgs<S, T> *)b->d_args[par][rec];
+    const int R = (int)b->runs.size();
     BHIP(hipStreamWaitEvent(b->st, b->evB, 0));
     BHIP(launch_bulk_batch<S, T>(args, b->d_tiles, b->ntiles, b->runs[0]->variant, b->grid, b->st));
     BHIP(hipStreamWaitEvent(b->st2, b->evA, 0));
-    BHIP(launch_side_batch<S, T>(args, (int)b->runs.size(), b->max_ring, b->max_nU, b->st2));
+    BHIP(launch_side_batch<S, T>(args, R, b->max_side[0], b->any_wall, b->st2));
     BHIP(hipEventRecord(b->evB, b->st2));
     BHIP(hipEventRecord(b->evA, b->st));
     return 0;
+}
+
+// Two steps of every run from parity par: the two-step bulk on stream A, the two band
+// sub-steps on stream B; the runs end at parity par ^ 1.
+template <class S, class T> static int launch_pair(mrlbm_batch *b, int par, int rec)
+{
+    const int R = (int)b->runs.size();
+    BHIP(hipStreamWaitEvent(b->st, b->evB, 0));
+    BHIP(launch_bulk2_batch<S, T>((const StepArgs<S, T> *)b->d_b2[par], b->d_tiles2, b->ntiles2, b->tb, b->st));
+    BHIP(hipStreamWaitEvent(b->st2, b->evA, 0));
+    BHIP(launch_side_batch<S, T>((const StepArgs<S, T> *)b->d_s1[par][rec], R, b->max_side[1], b->any_wall, b->st2));
+    BHIP(launch_side_batch<S, T>((const StepArgs<S, T> *)b->d_s2[par][rec], R, b->max_side[2], b->any_wall, b->st2));
+    BHIP(hipEventRecord(b->evB, b->st2));
+    BHIP(hipEventRecord(b->evA, b->st));
+    return 0;
+}
+
+// graph_steps steps from parity par: pairs (two-step mode) or single steps
+template <class S, class T> static int launch_steps(mrlbm_batch *b, int par, int rec, int n, int *par_out)
+{
+    int rc = 0, s = 0;
+    while (s < n && !rc) {
+        if (b->tb && n - s >= 2) {
+            rc = launch_pair<S, T>(b, par, rec);
+            s += 2;
+        } else {
+            rc = launch_step<S, T>(b, par, rec);
+            s += 1;
+        }
+        par ^= 1;
+    }
+    if (par_out) *par_out = par;
+    return rc;
 }
 
 template <class S, class T> static int capture(mrlbm_batch *b, int par, int rec)
@@ -124,7 +220,8 @@ template <class S, class T> static int capture(mrlbm_batch *b, int par, int rec)
     if (e == hipSuccess) e = hipStreamWaitEvent(b->st2, b->evA, 0);
     if (e == hipSuccess) e = hipEventRecord(b->evB, b->st2);
     int rc = e == hipSuccess ? 0 : MRLBM_E_HIP;
-    for (int s = 0; s < b->graph_steps && !rc; s++) rc = launch_step<S, T>(b, par ^ (s & 1), rec);
+    int par_end = par;
+    if (!rc) rc = launch_steps<S, T>(b, par, rec, b->graph_steps, &par_end);
     if (!rc && hipStreamWaitEvent(b->st, b->evB, 0) != hipSuccess) rc = MRLBM_E_HIP;
     const hipError_t ee = hipStreamEndCapture(b->st, &g);
     if (rc || ee != hipSuccess) {
@@ -134,6 +231,7 @@ template <class S, class T> static int capture(mrlbm_batch *b, int par, int rec)
     e = hipGraphInstantiate(&b->exec[par][rec], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     BHIP(e);
+    b->exec_flip = par_end ^ par;
     return 0;
 }
 
@@ -158,12 +256,13 @@ template <class S, class T> static int step_typed(mrlbm_batch *b, int nsteps, in
             BHIP(hipGraphLaunch(b->exec[par][rec], b->st));
             BHIP(hipEventRecord(b->evA, b->st));
             BHIP(hipEventRecord(b->evB, b->st));
-            s += b->graph_steps; // graph_steps is even: the parity is unchanged
+            par ^= b->exec_flip;
+            s += b->graph_steps;
         } else {
-            const int rc = launch_step<S, T>(b, par, rec);
+            const int n = (b->tb && nsteps - s >= 2) ? 2 : 1;
+            const int rc = launch_steps<S, T>(b, par, rec, n, &par);
             if (rc) return rc;
-            par ^= 1;
-            s++;
+            s += n;
         }
     }
     BHIP(hipStreamWaitEvent(b->st, b->evB, 0));
@@ -172,8 +271,10 @@ template <class S, class T> static int step_typed(mrlbm_batch *b, int nsteps, in
     return 0;
 }
 
-// Times every candidate on a saved copy of the states, keeps the fastest.
-template <class S, class T> static int autotune_typed(mrlbm_batch *b, int fixed_kernel, int fixed_persistent)
+// Times every candidate on a saved copy of the states, keeps the fastest: the single-step
+// bulk variants x grid modes, then two steps per pass with the strip heights in `tbs`.
+template <class S, class T>
+static int autotune_typed(mrlbm_batch *b, int fixed_kernel, int fixed_persistent, int fixed_tb)
 {
     std::vector<int> kernels;
     if (fixed_kernel) kernels = {fixed_kernel};
@@ -182,7 +283,12 @@ template <class S, class T> static int autotune_typed(mrlbm_batch *b, int fixed_
     std::vector<int> modes;
     if (fixed_persistent >= 0) modes = {fixed_persistent};
     else modes = {1, 0};
-    const bool tune = kernels.size() * modes.size() > 1;
+    std::vector<int> tbs;
+    if (fixed_tb > 0 && b->tb_possible) tbs = {fixed_tb};
+    else if (fixed_tb < 0 && b->tb_possible) tbs = b->f32 ? std::vector<int>{4, 8} : std::vector<int>{2, 4};
+    const bool only_tb = fixed_tb > 0 && b->tb_possible;
+    const size_t ncand = (only_tb ? 1 : kernels.size() * modes.size()) + tbs.size();
+    const bool tune = ncand > 1;
     std::vector<void *> saved;
     const int par0 = b->runs[0]->cur;
     if (tune)
@@ -198,69 +304,92 @@ template <class S, class T> static int autotune_typed(mrlbm_batch *b, int fixed_
         }
     b->tune_kernel.clear();
     b->tune_persistent.clear();
+    b->tune_tb.clear();
     b->tune_ms.clear();
     BHIP(hipEventRecord(b->evA, b->st));
     BHIP(hipEventRecord(b->evB, b->st));
-    int best_k = kernels[0], best_p = modes[0];
+    int best_k = kernels[0], best_p = modes[0], best_tb = only_tb ? tbs[0] : 0;
     double best = 1e300;
     int rc = 0;
-    for (int k : kernels) {
-        for (int pm : modes) {
-            std::string why;
-            for (mrlbm_handle *h : b->runs)
-                if (!rc) rc = set_bulk_shape(h, k, 0, why);
-            if (rc) break;
-            b->persistent = pm;
-            rc = build_tables<S, T>(b);
-            if (rc || !tune) break;
-            const int par = b->runs[0]->cur;
-            for (int s = 0; s < 2 && !rc; s++) rc = launch_step<S, T>(b, par ^ (s & 1), 0);
-            if (!rc && hipEventRecord(b->e0, b->st) != hipSuccess) rc = MRLBM_E_HIP;
-            for (int s = 0; s < b->tune_steps && !rc; s++) rc = launch_step<S, T>(b, par ^ (s & 1), 0);
-            if (!rc && hipStreamWaitEvent(b->st, b->evB, 0) != hipSuccess) rc = MRLBM_E_HIP;
-            if (!rc && hipEventRecord(b->e1, b->st) != hipSuccess) rc = MRLBM_E_HIP;
-            if (!rc && hipEventSynchronize(b->e1) != hipSuccess) rc = MRLBM_E_HIP;
-            float ms = 0;
-            if (!rc && hipEventElapsedTime(&ms, b->e0, b->e1) != hipSuccess) rc = MRLBM_E_HIP;
-            if (rc) break;
-            const double per = ms / b->tune_steps;
-            b->tune_kernel.push_back(k);
-            b->tune_persistent.push_back(pm);
-            b->tune_ms.push_back(per);
-            if (per < best) {
-                best = per;
-                best_k = k;
-                best_p = pm;
-            }
-            // restore the states and parity for the next candidate
-            for (size_t r = 0; r < b->runs.size(); r++) {
-                mrlbm_handle *h = b->runs[r];
-                h->cur = par0;
-                if (hipMemcpy(h->S[par0], saved[r], (size_t)6 * h->plane * sizeof(S), hipMemcpyDeviceToDevice) != hipSuccess)
-                    rc = MRLBM_E_HIP;
-            }
+    auto try_candidate = [&](int k, int pm, int tbc) -> int {
+        std::string why;
+        for (mrlbm_handle *h : b->runs) {
+            const int r2 = set_bulk_shape(h, k, 0, why);
+            if (r2) BFAIL(r2, why);
         }
-        if (rc || !tune) break;
-    }
+        b->persistent = pm;
+        b->tb = tbc;
+        int r3 = build_tables<S, T>(b);
+        if (r3 || !tune) return r3;
+        const int par = b->runs[0]->cur;
+        int pe = par;
+        r3 = launch_steps<S, T>(b, par, 0, 2, &pe);
+        if (!r3 && hipEventRecord(b->e0, b->st) != hipSuccess) r3 = MRLBM_E_HIP;
+        if (!r3) r3 = launch_steps<S, T>(b, pe, 0, b->tune_steps, &pe);
+        if (!r3 && hipStreamWaitEvent(b->st, b->evB, 0) != hipSuccess) r3 = MRLBM_E_HIP;
+        if (!r3 && hipEventRecord(b->e1, b->st) != hipSuccess) r3 = MRLBM_E_HIP;
+        if (!r3 && hipEventSynchronize(b->e1) != hipSuccess) r3 = MRLBM_E_HIP;
+        float ms = 0;
+        if (!r3 && hipEventElapsedTime(&ms, b->e0, b->e1) != hipSuccess) r3 = MRLBM_E_HIP;
+        if (r3) return r3;
+        const double per = ms / b->tune_steps;
+        b->tune_kernel.push_back(k);
+        b->tune_persistent.push_back(pm);
+        b->tune_tb.push_back(tbc);
+        b->tune_ms.push_back(per);
+        if (per < best) {
+            best = per;
+            best_k = k;
+            best_p = pm;
+            best_tb = tbc;
+        }
+        // restore the states and parity for the next candidate
+        for (size_t r = 0; r < b->runs.size(); r++) {
+            mrlbm_handle *h = b->runs[r];
+            h->cur = par0;
+            if (hipMemcpy(h->S[par0], saved[r], (size_t)6 * h->plane * sizeof(S), hipMemcpyDeviceToDevice) != hipSuccess)
+                return MRLBM_E_HIP;
+        }
+        return 0;
+    };
+    if (!only_tb)
+        for (int k : kernels) {
+            for (int pm : modes) {
+                rc = try_candidate(k, pm, 0);
+                if (rc || !tune) break;
+            }
+            if (rc || !tune) break;
+        }
+    const int single_k = best_k, single_p = best_p; // the remainder steps of a two-step run
+    if (!rc && tune)
+        for (int tbc : tbs) {
+            rc = try_candidate(single_k, only_tb ? modes[0] : single_p, tbc);
+            if (rc) break;
+        }
     for (void *q : saved) (void)hipFree(q);
     if (rc) return rc;
     b->kernel = best_k;
-    b->persistent = best_p;
     std::string why;
     for (mrlbm_handle *h : b->runs) {
         const int r2 = set_bulk_shape(h, best_k, 0, why);
         if (r2) BFAIL(r2, why);
     }
+    b->persistent = best_p;
+    b->tb = best_tb;
     return build_tables<S, T>(b);
 }
 
 static void batch_free(mrlbm_batch *b)
 {
     drop_graphs(b);
-    for (auto &row : b->d_args)
-        for (void *p : row)
-            if (p) (void)hipFree(p);
+    for (int par = 0; par < 2; par++) {
+        for (int rec = 0; rec < 2; rec++)
+            for (void *p : {b->d_args[par][rec], b->d_s1[par][rec], b->d_s2[par][rec]})
+                if (p) (void)hipFree(p);
+        if (b->d_b2[par]) (void)hipFree(b->d_b2[par]);
+    }
     if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->d_tiles2) (void)hipFree(b->d_tiles2);
     if (b->d_facc) (void)hipFree(b->d_facc);
     for (hipEvent_t e : {b->evA, b->evB, b->e0, b->e1})
         if (e) (void)hipEventDestroy(e);
@@ -277,6 +406,7 @@ extern "C" void mrlbm_default_batch_opts(mrlbm_batch_opts *o)
     o->persistent = -1;
     o->graph_steps = 50;
     o->tune_steps = 20;
+    o->time_blocking = -1;
 }
 
 extern "C" int mrlbm_batch_create(const mrlbm_params *runs, int32_t nruns, const mrlbm_batch_opts *o,
@@ -301,6 +431,9 @@ extern "C" int mrlbm_batch_create(const mrlbm_params *runs, int32_t nruns, const
         return code;
     };
     if (opts.graph_steps % 2) return fail(MRLBM_E_ARG, "graph_steps must be even");
+    if (opts.time_blocking != -1 && opts.time_blocking != 0 && opts.time_blocking != 2 && opts.time_blocking != 4 &&
+        opts.time_blocking != 8)
+        return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4 or 8");
     if (opts.kernel == 1) return fail(MRLBM_E_ARG, "the batched bulk has no one-node-per-thread kernel");
     b->graph_steps = opts.graph_steps;
     b->tune_steps = opts.tune_steps;
@@ -309,11 +442,14 @@ extern "C" int mrlbm_batch_create(const mrlbm_params *runs, int32_t nruns, const
         if ((runs[r].precision == MRLBM_FP32_DEV) != b->f32) return fail(MRLBM_E_ARG, "every run needs the same precision");
         mrlbm_handle *h = nullptr;
         mrlbm_params pr = runs[r];
-        pr.time_blocking = 0; // the batch steps its runs with its own launches
+        // any strip height: it allocates the scratch state and the band rectangles of the
+        // two-step pass (the batch picks the strip height itself)
+        pr.time_blocking = opts.time_blocking == 0 ? 0 : 2;
         const int rc = mrlbm_create(&pr, &h);
         if (rc) return fail(rc, "run " + std::to_string(r) + ": " + mrlbm_last_error(nullptr));
         b->runs.push_back(h);
         b->nodes += (int64_t)h->nx * h->ny;
+        if (!h->S2) b->tb_possible = false;
         if (!h->d_rec) { // force reduction target of the boundary workgroup
             h->rec_stride = 14 + 2 * h->bt.nbct;
             h->rec_cap = 256;
@@ -329,8 +465,8 @@ extern "C" int mrlbm_batch_create(const mrlbm_params *runs, int32_t nruns, const
     if (hipMalloc(&b->d_facc, (size_t)4 * nruns * sizeof(double)) != hipSuccess ||
         hipMemset(b->d_facc, 0, (size_t)4 * nruns * sizeof(double)) != hipSuccess)
         return fail(MRLBM_E_HIP, "force sums");
-    const int rc = b->f32 ? autotune_typed<float, float>(b, opts.kernel, opts.persistent)
-                          : autotune_typed<double, double>(b, opts.kernel, opts.persistent);
+    const int rc = b->f32 ? autotune_typed<float, float>(b, opts.kernel, opts.persistent, opts.time_blocking)
+                          : autotune_typed<double, double>(b, opts.kernel, opts.persistent, opts.time_blocking);
     if (rc) return fail(rc, "");
     *out = b;
     return MRLBM_OK;
@@ -358,7 +494,8 @@ extern "C" mrlbm_handle *mrlbm_batch_run(mrlbm_batch *b, int32_t r)
 extern "C" int mrlbm_batch_autotune(mrlbm_batch *b)
 {
     if (!b) return MRLBM_E_ARG;
-    return b->f32 ? autotune_typed<float, float>(b, 0, -1) : autotune_typed<double, double>(b, 0, -1);
+    const int tbc = b->tb_possible ? -1 : 0;
+    return b->f32 ? autotune_typed<float, float>(b, 0, -1, tbc) : autotune_typed<double, double>(b, 0, -1, tbc);
 }
 
 extern "C" int mrlbm_batch_step(mrlbm_batch *b, int32_t nsteps, int32_t record_forces)
@@ -403,10 +540,12 @@ extern "C" int mrlbm_batch_get_info(const mrlbm_batch *b, mrlbm_batch_info *info
     info->ntiles = b->ntiles;
     info->graph_steps = b->graph_steps;
     info->nodes = b->nodes;
+    info->time_blocking = b->tb;
     info->ntune = (int32_t)std::min<size_t>(16, b->tune_ms.size());
     for (int i = 0; i < info->ntune; i++) {
         info->tune_kernel[i] = b->tune_kernel[i];
         info->tune_persistent[i] = b->tune_persistent[i];
+        info->tune_tb[i] = b->tune_tb[i];
         info->tune_ms[i] = b->tune_ms[i];
     }
     return MRLBM_OK;

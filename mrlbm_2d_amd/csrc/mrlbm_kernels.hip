@@ -444,9 +444,8 @@ template <class S, class T> __global__ void __launch_bounds__(256) band_kernel(S
 // the band rectangles of a two-step pair (plain update) and the curved wall's stencil
 // nodes (pre-collision moments for boundary_kernel).  All three read only the source
 // state and write disjoint nodes.
-template <class S, class T> __global__ void __launch_bounds__(256) side_kernel(StepArgs<S, T> a, int nring, int nband)
+template <class S, class T> __device__ void side_body(const StepArgs<S, T> &a, long long t, int nring, int nband)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nring) {
         edge_body(a, (int)t);
         return;
@@ -468,6 +467,18 @@ template <class S, class T> __global__ void __launch_bounds__(256) side_kernel(S
     }
     t -= nband;
     if (a.has_boundary) stencil_moments_body(a, (int)t);
+}
+
+template <class S, class T> __device__ __forceinline__ int band_nodes(const StepArgs<S, T> &a)
+{
+    int n = 0;
+    for (int q = 0; q < a.nrect; q++) n += a.rect[q][2] * a.rect[q][3];
+    return n;
+}
+
+template <class S, class T> __global__ void __launch_bounds__(256) side_kernel(StepArgs<S, T> a, int nring, int nband)
+{
+    side_body(a, (long long)blockIdx.x * blockDim.x + threadIdx.x, nring, nband);
 }
 
 // Bulk, one node per thread with the generic gather: the first step after
@@ -688,9 +699,9 @@ template <class S, class T, bool REC, int H> __global__ void __launch_bounds__(2
 
 // ---------------------------------------------------------------------------------
 // Batched runs: per-run argument blocks in device memory, one launch per kernel for all
-// runs.  The bulk walks a flattened tile table (run, block of that run) with a persistent
-// grid; the edge and stencil kernels take the run from blockIdx.y, the boundary workgroup
-// of run r is block r.
+// runs.  The bulk walks a flattened tile table (run, block of that run), persistent or one
+// block per tile; the side kernel takes the run from blockIdx.y, the boundary workgroup of
+// run r is block r.
 template <class S, class T, int H>
 __global__ void __launch_bounds__(256) bulk_batch_kernel(const StepArgs<S, T> *args, const int2 *tiles, int ntiles)
 {
@@ -701,24 +712,27 @@ __global__ void __launch_bounds__(256) bulk_batch_kernel(const StepArgs<S, T> *a
     }
 }
 
-template <class S, class T> __global__ void __launch_bounds__(256) edge_batch_kernel(const StepArgs<S, T> *args)
-{
-    const StepArgs<S, T> a = args[blockIdx.y];
-    edge_body(a, blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-template <class S, class T> __global__ void __launch_bounds__(256) stencil_batch_kernel(const StepArgs<S, T> *args)
-{
-    const StepArgs<S, T> &a = args[blockIdx.y];
-    if (!a.has_boundary) return;
-    stencil_moments_body(a, blockIdx.x * blockDim.x + threadIdx.x);
-}
-
 template <class S, class T> __global__ void __launch_bounds__(256) boundary_batch_kernel(const StepArgs<S, T> *args)
 {
     const StepArgs<S, T> &a = args[blockIdx.x];
     if (!a.has_boundary) return;
     boundary_body(a);
+}
+
+template <class S, class T> __global__ void __launch_bounds__(256) side_batch_kernel(const StepArgs<S, T> *args)
+{
+    const StepArgs<S, T> &a = args[blockIdx.y];
+    side_body(a, (long long)blockIdx.x * blockDim.x + threadIdx.x, 2 * a.nx + 2 * (a.ny - 2), band_nodes(a));
+}
+
+template <class S, class T, int H>
+__global__ void __launch_bounds__(256) bulk2_batch_kernel(const StepArgs<S, T> *args, const int2 *tiles, int ntiles)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        const StepArgs<S, T> a = args[tl.x];
+        bulk_march2<S, T, H>(a, tl.y);
+    }
 }
 
 template <class S, class T> __global__ void __launch_bounds__(256) bulk_simple_kernel(StepArgs<S, T> a)
@@ -852,12 +866,21 @@ template <class S, class T> int bulk_batch_resident_blocks(int variant)
     return per_cu * ncu;
 }
 template <class S, class T>
-hipError_t launch_side_batch(const StepArgs<S, T> *args, int nruns, int max_ring, int max_nU, hipStream_t st)
+hipError_t launch_side_batch(const StepArgs<S, T> *args, int nruns, int max_side, int any_wall, hipStream_t st)
 {
-    hipLaunchKernelGGL((edge_batch_kernel<S, T>), dim3((max_ring + 255) / 256, nruns), dim3(256), 0, st, args);
-    if (max_nU > 0) {
-        hipLaunchKernelGGL((stencil_batch_kernel<S, T>), dim3((max_nU + 255) / 256, nruns), dim3(256), 0, st, args);
-        hipLaunchKernelGGL((boundary_batch_kernel<S, T>), dim3(nruns), dim3(256), 0, st, args);
+    hipLaunchKernelGGL((side_batch_kernel<S, T>), dim3((max_side + 255) / 256, nruns), dim3(256), 0, st, args);
+    if (any_wall) hipLaunchKernelGGL((boundary_batch_kernel<S, T>), dim3(nruns), dim3(256), 0, st, args);
+    return hipGetLastError();
+}
+template <class S, class T>
+hipError_t launch_bulk2_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int H, hipStream_t st)
+{
+    const dim3 g(ntiles), b(256);
+    switch (H) {
+    case 2: hipLaunchKernelGGL((bulk2_batch_kernel<S, T, 2>), g, b, 0, st, args, tiles, ntiles); break;
+    case 4: hipLaunchKernelGGL((bulk2_batch_kernel<S, T, 4>), g, b, 0, st, args, tiles, ntiles); break;
+    case 8: hipLaunchKernelGGL((bulk2_batch_kernel<S, T, 8>), g, b, 0, st, args, tiles, ntiles); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -933,6 +956,7 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_wall<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
     template hipError_t launch_bulk_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, int, hipStream_t); \
     template hipError_t launch_side_batch<S, T>(const StepArgs<S, T> *, int, int, int, hipStream_t); \
+    template hipError_t launch_bulk2_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, hipStream_t); \
     template int bulk_batch_resident_blocks<S, T>(int);                                          \
     template hipError_t launch_inlet_fold<S, T>(const StepArgs<S, T> &, hipStream_t);            \
     template hipError_t launch_get_populations<S, T>(const StepArgs<S, T> &, double *, hipStream_t); \

@@ -37,17 +37,23 @@ def single(p, run, n1, n2):
 
 
 @pytest.mark.parametrize("precision", [capi.MRLBM_FP32_DEV, capi.MRLBM_FP64])
-@pytest.mark.parametrize("kernel,persistent", [(0, -1), (8, 0), (304, 1), (2, 1)])
-def test_batch_matches_single_runs(precision, kernel, persistent):
+@pytest.mark.parametrize("kernel,persistent,time_blocking", [(0, -1, -1), (8, 0, 0), (304, 1, 0), (2, 1, 0),
+                                                             (304, 0, 4), (8, 0, 2), (16, 1, 8)])
+def test_batch_matches_single_runs(precision, kernel, persistent, time_blocking):
     ps = runs(precision)
     o = capi.default_batch_opts()
-    o.kernel, o.persistent, o.graph_steps = kernel, persistent, 20
-    n1, n2 = 41, 60  # 41: two graphs and one direct step, so the second call starts at odd parity
+    o.kernel, o.persistent, o.graph_steps, o.time_blocking = kernel, persistent, 22, time_blocking
+    # 22-step graphs: 41 = one graph + direct steps (a pair and a single step in two-step
+    # mode), so the second call starts at the other parity; 60 = two graphs + 16 direct
+    n1, n2 = 41, 60
     with capi.Batch(ps, o) as b:
         info = b.info
-        print(f"[gpu] batch prec {precision} kernel {info.kernel} persistent {info.persistent} grid {info.grid} "
-              f"tiles {info.ntiles} tuned {[(info.tune_kernel[i], info.tune_persistent[i], round(info.tune_ms[i], 4)) for i in range(info.ntune)]}",
+        print(f"[gpu] batch prec {precision} kernel {info.kernel} persistent {info.persistent} time_blocking "
+              f"{info.time_blocking} grid {info.grid} tiles {info.ntiles} tuned "
+              f"{[(info.tune_kernel[i], info.tune_persistent[i], info.tune_tb[i], round(info.tune_ms[i], 4)) for i in range(info.ntune)]}",
               flush=True)
+        if time_blocking > 0:
+            assert info.time_blocking == time_blocking
         if kernel == 0:
             assert info.ntune >= 2
             best = min(range(info.ntune), key=lambda i: info.tune_ms[i])
