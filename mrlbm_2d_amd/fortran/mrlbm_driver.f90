@@ -218,9 +218,29 @@ program mrlbm_gpu
     if (post_process) then
         close(101); close(102); close(103)
     end if
+    if (post_process .and. channel_with_cylinder) call write_input_stat()
     rc = mrlbm_destroy(h)
 
 contains
+
+    ! The post-processor's input for this run, in the input_stat.dat format
+    ! (2d_statistics.f90:29-32): boundary-node count, cylinder radius, inlet velocity and the
+    ! running mean of the inlet density.
+    subroutine write_input_stat()
+        type(mrlbm_info) :: inf
+        rc = mrlbm_get_info(h, inf)
+        call check(rc)
+        rc = mrlbm_get_rho_inf_mean(h, rho_inf_mean)
+        call check(rc)
+        open(unit=311, file='data_mean/input_stat.dat', status='replace')
+            write(311, '(a)') '&parameters'
+            write(311, '(a,i0)') '    nbct = ', inf%nbct
+            write(311, '(a,es25.17)') '    r_cy = ', inf%r_cyl
+            write(311, '(a,es25.17)') '    uo = ', uo
+            write(311, '(a,es25.17)') '    rho_infty = ', rho_inf_mean
+            write(311, '(a)') '/'
+        close(311)
+    end subroutine
 
     subroutine check(code)
         integer(c_int), intent(in) :: code

@@ -16,7 +16,8 @@ Drop-in for the reference's post-processing program, run in a run directory:
 
     python -m mrlbm_2d_amd.statistics [run_dir]
 
-reads input_stat.dat, data_mean/forces_time.dat, theta.dat and the unformatted
+reads input_stat.dat (or data_mean/input_stat.dat, which the drop-in driver writes from the
+run's nbct, r_cyl, uo and inlet-density mean), data_mean/forces_time.dat, theta.dat and the unformatted
 p_surface{,_smooth}_time.dat, and writes plots/cp.dat, cp_fit.dat, cd_time.dat,
 cl_time.dat and drag_lift.dat with the same rows (2d_statistics.f90:29-178).
 """
@@ -125,7 +126,10 @@ def _fmt(v):
 def run(directory="."):
     """The reference post-processor on the files of one run directory."""
     from .namelist import read_input_stat
-    prm = read_input_stat(os.path.join(directory, "input_stat.dat"))
+    path = os.path.join(directory, "input_stat.dat")
+    if not os.path.exists(path):  # the one the drop-in driver writes at the end of a run
+        path = os.path.join(directory, "data_mean", "input_stat.dat")
+    prm = read_input_stat(path)
     nbct, r_cy, uo, rho_infty = int(prm["nbct"]), float(prm["r_cy"]), float(prm["uo"]), float(prm["rho_infty"])
     dm = os.path.join(directory, "data_mean")
     rows = [l.split() for l in open(os.path.join(dm, "forces_time.dat")) if l.strip()]

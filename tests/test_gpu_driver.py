@@ -59,3 +59,11 @@ def test_driver_files_match_reference(tmp_path, name):
         np.testing.assert_allclose(theta, g["theta"], rtol=0, atol=1e-15)
         rm = np.atleast_2d(np.loadtxt(os.path.join(d, "rho_mean_inlet.dat")))
         np.testing.assert_allclose(rm, g["rho_mean_inlet"], rtol=1e-14)
+        # the post-processor's input written for this run
+        from mrlbm_2d_amd.namelist import read_input_stat
+        st = read_input_stat(os.path.join(d, "data_mean", "input_stat.dat"))
+        assert int(st["nbct"]) == len(g["theta"])
+        np.testing.assert_allclose(float(st["uo"]), float(g["params"]["uo"]), rtol=1e-15)
+        # the running mean after the last step (the file's last row may be a few steps older)
+        np.testing.assert_allclose(float(st["rho_infty"]), g["rho_mean_inlet"][-1, 1], rtol=1e-3)
+        assert float(st["r_cy"]) > 0
