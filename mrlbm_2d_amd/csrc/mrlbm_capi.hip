@@ -310,6 +310,15 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
     if (p->precision != MRLBM_FP64 && p->precision != MRLBM_FP32_DEV) return fail(MRLBM_E_ARG, "unknown precision");
     if (p->mass_bc < 0 || p->mass_bc > 2) return fail(MRLBM_E_ARG, "unknown mass_bc");
     if ((long long)p->nx * p->ny > (1LL << 31) - 1) return fail(MRLBM_E_ARG, "grid too large for 32-bit node index");
+    {
+        const int k = p->kernel;
+        if (k != 0 && k != 1 && k != 2 && k != 4 && k != 8 && k != 16 && k != 304 && k != 308 && k != 316)
+            return fail(MRLBM_E_ARG, "kernel must be 0, 1, 2, 4, 8, 16, 304, 308 or 316");
+        const int tb = p->time_blocking;
+        if (tb != -1 && tb != 0 && tb != 2 && tb != 4 && tb != 8)
+            return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4 or 8");
+        if (p->strip_rows < 0 || p->strip_rows > 64) return fail(MRLBM_E_ARG, "strip_rows must be 0..64");
+    }
     h->nx = p->nx;
     h->ny = p->ny;
     h->pitch = (p->nx + 63) / 64 * 64;

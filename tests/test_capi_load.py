@@ -41,3 +41,22 @@ def test_create_reports_bad_arguments_without_gpu():
         assert e.code == capi.E_ARG
     else:
         raise AssertionError("expected an error")
+
+
+def test_create_rejects_bad_bulk_options_without_gpu():
+    for field, value in (("kernel", 3), ("kernel", 305), ("time_blocking", 3), ("time_blocking", -2),
+                         ("strip_rows", 65), ("precision", 7), ("mass_bc", 5)):
+        p = capi.default_params()
+        p.nx, p.ny = 64, 32
+        setattr(p, field, value)
+        try:
+            capi.Lattice(p)
+        except capi.MrlbmError as e:
+            assert e.code == capi.E_ARG, (field, value, e)
+        else:
+            raise AssertionError(f"{field}={value} accepted")
+
+
+def test_default_params_pick_the_measured_variants():
+    p = capi.default_params()
+    assert p.kernel == 0 and p.time_blocking == -1 and p.block_order == 0
