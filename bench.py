@@ -8,7 +8,8 @@ two steps per launch of the bulk kernel (temporal blocking), bit-identical to si
 
 Prints ONE JSON line.  `roofline` prices the dominant kernel against MI355X's 8 TB/s HBM:
 achieved = algorithmic bytes per launch (12 moment words per node and step: 6 read + 6
-written, two steps) / mean launch time measured with HIP events on the library's stream;
+written, two steps) / mean launch time, from a HIP event pair around each launch on the
+bulk kernel's own stream;
 traffic = the launch's measured HBM bytes (PMC), below the algorithmic bytes because the
 intermediate step never leaves the chip.  Also: the 512x128 drag adjoint (adjoint / forward
 time) and 256 batched runs (aggregate MLUPS).
@@ -32,12 +33,14 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel at 8192 x 2048 (the two-step bulk pass, two
-# steps per launch: FP32 strips of 4 rows, FP64 of 2), from rocprofv3 FETCH_SIZE and
+# steps per launch: FP32 strips of 10 rows, FP64 of 8), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1);
 # the summary and the commands are in profiles/bulk2_kernel_8192x2048_hbm_bytes.md
-TRAFFIC_BYTES = {"f32": 901_869_006, "f64": 1_706_013_162}
-STEPS_PER_LAUNCH = 2
+TRAFFIC_BYTES = {"f32": 947_831_915, "f64": 1_797_906_848}
 NX, NY = 8192, 2048
+# HIP event pairs around every 4th bulk launch: the per-launch mean without the events' own
+# cost (about 2% of the step when recorded around every launch)
+TIMED_EVERY = 4
 XC, YC, RC = 1024.0, 1025.0, 64.0
 UMAX, RE = 0.05, 200.0
 
@@ -66,23 +69,27 @@ def time_form(precision, steps, warmup, **kw):
     lat = make_lattice(precision, **kw)
     lat.step(2, warmup)
     lat.synchronize()
-    lat.set_timing(True)
+    lat.set_timing(TIMED_EVERY)
     t0 = time.perf_counter()
     lat.step(2 + warmup, steps)
     lat.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = lat.last_step_ms()
+    step_ms = lat.last_step_ms()
+    launch_ms, launches = lat.last_bulk_ms()
     assert lat.check_finite() == 0, "non-finite state after the timed steps"
     info = lat.info
     lat.close()
     n = info.nx * info.ny
     mlups = n * steps / (wall * 1e6)
-    # kern_ms: mean time per step between HIP events on the bulk kernel's stream over the
-    # timed steps (the side stream is joined into it); one launch covers two steps
+    # launch_ms: mean duration of one bulk launch, from HIP event pairs around every
+    # TIMED_EVERY-th launch on the bulk kernel's own stream (launches timed = ceil(all launches
+    # / TIMED_EVERY)); a two-step launch covers two steps
+    steps_per_launch = round(steps / (launches * TIMED_EVERY))
     bytes_per_step = int(info.bytes_per_step)
-    achieved = bytes_per_step / (kern_ms * 1e-3) / 1e9
-    return dict(mlups=mlups, wall_s=wall, ms_per_step=1e3 * wall / steps, kernel_ms=kern_ms,
-                bytes_per_step=bytes_per_step, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS)
+    achieved = steps_per_launch * bytes_per_step / (launch_ms * 1e-3) / 1e9
+    return dict(mlups=mlups, wall_s=wall, ms_per_step=1e3 * wall / steps, step_ms=step_ms,
+                launch_ms=launch_ms, steps_per_launch=steps_per_launch, bytes_per_step=bytes_per_step,
+                achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS)
 
 
 def roofline(r, kernel, traffic):
@@ -91,9 +98,9 @@ def roofline(r, kernel, traffic):
     of one launch (PMC)."""
     return {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(r["frac"], 4), "traffic": traffic, "kernel": kernel,
-            "steps_per_launch": STEPS_PER_LAUNCH,
-            "algorithmic_bytes_per_launch": STEPS_PER_LAUNCH * r["bytes_per_step"],
-            "launch_ms": round(STEPS_PER_LAUNCH * r["kernel_ms"], 5)}
+            "steps_per_launch": r["steps_per_launch"],
+            "algorithmic_bytes_per_launch": r["steps_per_launch"] * r["bytes_per_step"],
+            "launch_ms": round(r["launch_ms"], 5)}
 
 
 def cpu_baseline(nsteps=2):
@@ -270,9 +277,9 @@ def main():
         "data": "synthetic (seeded +-1e-4 velocity noise)",
         "config": {"workload": "8192x2048 channel, cylinder D=128 at (1024,1025), "
                                "parabolic inlet u_max=0.05, Re=200", "nx": NX, "ny": NY},
-        "roofline": roofline(r32, "bulk2_kernel<float,float,4>", TRAFFIC_BYTES["f32"] if default_shape else None),
+        "roofline": roofline(r32, "bulk2_kernel<float,float,10>", TRAFFIC_BYTES["f32"] if default_shape else None),
         "fp64": {"value": round(r64["mlups"], 1), "ms_per_step": round(r64["ms_per_step"], 5),
-                 "roofline": roofline(r64, "bulk2_kernel<double,double,2>",
+                 "roofline": roofline(r64, "bulk2_kernel<double,double,8>",
                                       TRAFFIC_BYTES["f64"] if default_shape else None)},
         "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3),
         "adjoint": adj,

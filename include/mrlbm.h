@@ -61,10 +61,10 @@ typedef struct mrlbm_params {
                                      of that many rows with 62 written columns per wave, 304/308/316
                                      aligned 64-column waves with an unrolled strip of kernel-300 rows */
     int32_t block_order;          /* bulk blocks: 0 plain, 1 XCD-aware (vertical neighbours share an L2) */
-    int32_t time_blocking;        /* 0: one bulk pass per step; 2/4/8: two steps per bulk pass over the
-                                     plain interior (strips of that many rows), the edges and the
-                                     cylinder band in two ordinary sub-steps; -1 (default) auto:
-                                     4 in FP32, 2 in FP64.  Same bits either way */
+    int32_t time_blocking;        /* 0: one bulk pass per step; 2..12 (even): two steps per bulk pass
+                                     over the plain interior (strips of that many rows), the edges
+                                     and the cylinder band in two ordinary sub-steps; -1 (default)
+                                     auto: 10 rows FP32, 8 rows FP64.  Same bits either way */
     int32_t reserved[4];
 } mrlbm_params;
 
@@ -211,7 +211,7 @@ typedef struct mrlbm_batch_opts {
                                      0 one block per tile, -1 autotune */
     int32_t graph_steps;          /* steps per captured graph (even; 0 = 50) */
     int32_t tune_steps;           /* timed steps per autotune candidate (0 = 20) */
-    int32_t time_blocking;        /* -1 autotune, 0 one step per pass, 2/4/8 two steps per pass
+    int32_t time_blocking;        /* -1 autotune, 0 one step per pass, 2..10 (even) two steps per pass
                                      with strips of that many rows (as mrlbm_params) */
     int32_t reserved[3];
 } mrlbm_batch_opts;
@@ -244,10 +244,17 @@ int mrlbm_batch_forces(mrlbm_batch *b, double *fd_sum, double *fl_sum, double *f
 int mrlbm_batch_reset_forces(mrlbm_batch *b);
 int mrlbm_batch_get_info(const mrlbm_batch *b, mrlbm_batch_info *info);
 
-/* Mean kernel time (ms) of the bulk step over the last mrlbm_step call, from HIP events
- * recorded on the handle's stream; 0 when timing is off. */
+/* Timing mode: on = 0 off; on = n >= 1 records a HIP event pair around every n-th bulk launch
+ * of each mrlbm_step call (the first included; n > 1 keeps the events' own cost out of most
+ * launches).  mrlbm_last_step_ms: mean time per step of the last mrlbm_step call between
+ * HIP events on the handle's stream (0 when timing is off). */
 int mrlbm_set_timing(mrlbm_handle *h, int32_t on);
 int mrlbm_last_step_ms(const mrlbm_handle *h, double *ms);
+/* Mean duration (ms) of one bulk-kernel launch over the last timed mrlbm_step call (a
+ * two-step launch covers two iterations), from the HIP event pairs of the timed launches on
+ * the bulk stream (at most 4096); *launches = number of launches timed.
+ * New: no counterpart in the reference (its loop prints MLUPS only, 2d_mrlbm.f90:540-553). */
+int mrlbm_last_bulk_ms(const mrlbm_handle *h, double *ms_per_launch, int32_t *launches);
 
 #ifdef __cplusplus
 }

@@ -27,7 +27,7 @@ EXPORTS = (
     "mrlbm_get_fields", "mrlbm_get_moments", "mrlbm_get_populations", "mrlbm_get_forces",
     "mrlbm_get_surface_p", "mrlbm_get_surface_p_series", "mrlbm_get_rho_inf_mean",
     "mrlbm_get_probes", "mrlbm_get_stats", "mrlbm_reset_stats", "mrlbm_max_ux",
-    "mrlbm_check_finite", "mrlbm_set_timing", "mrlbm_last_step_ms",
+    "mrlbm_check_finite", "mrlbm_set_timing", "mrlbm_last_step_ms", "mrlbm_last_bulk_ms",
     "mrlbm_set_brinkman", "mrlbm_default_adjoint_opts", "mrlbm_adjoint_drag",
     "mrlbm_default_batch_opts", "mrlbm_batch_create", "mrlbm_batch_destroy", "mrlbm_batch_last_error",
     "mrlbm_batch_run", "mrlbm_batch_autotune", "mrlbm_batch_step", "mrlbm_batch_forces",
@@ -148,6 +148,7 @@ def load(path: str = LIB_PATH):
     L.mrlbm_check_finite.argtypes = [vp]
     L.mrlbm_set_timing.argtypes = [vp, C.c_int32]
     L.mrlbm_last_step_ms.argtypes = [vp, dp]
+    L.mrlbm_last_bulk_ms.argtypes = [vp, dp, C.POINTER(C.c_int32)]
     L.mrlbm_set_brinkman.argtypes = [vp, dp]
     L.mrlbm_default_adjoint_opts.argtypes = [C.POINTER(AdjointOpts)]
     L.mrlbm_default_adjoint_opts.restype = None
@@ -358,13 +359,20 @@ class Lattice:
     def check_finite(self):
         return self._L.mrlbm_check_finite(self._h)
 
-    def set_timing(self, on: bool = True):
-        self._check(self._L.mrlbm_set_timing(self._h, 1 if on else 0))
+    def set_timing(self, on=True):
+        """False/0 off; True/1 time every bulk launch; n > 1 every n-th launch."""
+        self._check(self._L.mrlbm_set_timing(self._h, int(on)))
 
     def last_step_ms(self):
         v = C.c_double()
         self._check(self._L.mrlbm_last_step_ms(self._h, C.byref(v)))
         return v.value
+
+    def last_bulk_ms(self):
+        """(mean ms of one bulk launch, launches timed) over the last timed step call."""
+        v, n = C.c_double(), C.c_int32()
+        self._check(self._L.mrlbm_last_bulk_ms(self._h, C.byref(v), C.byref(n)))
+        return v.value, n.value
 
 
 def default_batch_opts() -> BatchOpts:

@@ -285,7 +285,7 @@ static int autotune_typed(mrlbm_batch *b, int fixed_kernel, int fixed_persistent
     else modes = {1, 0};
     std::vector<int> tbs;
     if (fixed_tb > 0 && b->tb_possible) tbs = {fixed_tb};
-    else if (fixed_tb < 0 && b->tb_possible) tbs = b->f32 ? std::vector<int>{4, 8} : std::vector<int>{2, 4};
+    else if (fixed_tb < 0 && b->tb_possible) tbs = b->f32 ? std::vector<int>{6, 10} : std::vector<int>{6, 8};
     const bool only_tb = fixed_tb > 0 && b->tb_possible;
     const size_t ncand = (only_tb ? 1 : kernels.size() * modes.size()) + tbs.size();
     const bool tune = ncand > 1;
@@ -431,9 +431,8 @@ extern "C" int mrlbm_batch_create(const mrlbm_params *runs, int32_t nruns, const
         return code;
     };
     if (opts.graph_steps % 2) return fail(MRLBM_E_ARG, "graph_steps must be even");
-    if (opts.time_blocking != -1 && opts.time_blocking != 0 && opts.time_blocking != 2 && opts.time_blocking != 4 &&
-        opts.time_blocking != 8)
-        return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4 or 8");
+    if (opts.time_blocking < -1 || opts.time_blocking > 10 || (opts.time_blocking > 0 && opts.time_blocking % 2))
+        return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4, 6, 8 or 10");
     if (opts.kernel == 1) return fail(MRLBM_E_ARG, "the batched bulk has no one-node-per-thread kernel");
     b->graph_steps = opts.graph_steps;
     b->tune_steps = opts.tune_steps;
@@ -458,8 +457,8 @@ extern "C" int mrlbm_batch_create(const mrlbm_params *runs, int32_t nruns, const
         }
     }
     if (hipStreamCreate(&b->st) != hipSuccess || hipStreamCreate(&b->st2) != hipSuccess ||
-        hipEventCreateWithFlags(&b->evA, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->evB, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->evA, hipEventDisableTiming | hipEventReleaseToDevice) != hipSuccess ||
+        hipEventCreateWithFlags(&b->evB, hipEventDisableTiming | hipEventReleaseToDevice) != hipSuccess ||
         hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess)
         return fail(MRLBM_E_HIP, "stream or event creation failed");
     if (hipMalloc(&b->d_facc, (size_t)4 * nruns * sizeof(double)) != hipSuccess ||

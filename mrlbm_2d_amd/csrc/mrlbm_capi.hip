@@ -189,10 +189,12 @@ static void free_all(mrlbm_handle *h)
     h->allocs.clear();
     if (h->e0) (void)hipEventDestroy(h->e0);
     if (h->e1) (void)hipEventDestroy(h->e1);
-    if (h->evA) (void)hipEventDestroy(h->evA);
-    if (h->evB) (void)hipEventDestroy(h->evB);
-    if (h->st) (void)hipStreamDestroy(h->st);
-    if (h->st2) (void)hipStreamDestroy(h->st2);
+    for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
+    h->tev.clear();
+    for (hipEvent_t e : {h->evA, h->evB, h->evS, h->evW})
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t q : {h->st, h->st2, h->st3})
+        if (q) (void)hipStreamDestroy(q);
 }
 
 int mrlbm::set_bulk_shape(mrlbm_handle *h, int kernel_in, int strip_rows, std::string &why)
@@ -315,8 +317,8 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
         if (k != 0 && k != 1 && k != 2 && k != 4 && k != 8 && k != 16 && k != 304 && k != 308 && k != 316)
             return fail(MRLBM_E_ARG, "kernel must be 0, 1, 2, 4, 8, 16, 304, 308 or 316");
         const int tb = p->time_blocking;
-        if (tb != -1 && tb != 0 && tb != 2 && tb != 4 && tb != 8)
-            return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4 or 8");
+        if (tb != -1 && tb != 0 && tb != 2 && tb != 4 && tb != 6 && tb != 8 && tb != 10 && tb != 12)
+            return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4, 6, 8, 10 or 12");
         if (p->strip_rows < 0 || p->strip_rows > 64) return fail(MRLBM_E_ARG, "strip_rows must be 0..64");
     }
     h->nx = p->nx;
@@ -338,11 +340,17 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
     else h->nu = uref * (double)(p->ny - 1) / p->Re;
     h->omega = p->omega > 0 ? p->omega : 1.0 / ((3.0 * h->nu) + (1.0 / 2.0));
     if (p->omega > 0) h->nu = (1.0 / p->omega - 0.5) / 3.0;
-    if (hipStreamCreate(&h->st) != hipSuccess || hipStreamCreate(&h->st2) != hipSuccess)
+    if (hipStreamCreate(&h->st) != hipSuccess || hipStreamCreate(&h->st2) != hipSuccess ||
+        hipStreamCreate(&h->st3) != hipSuccess)
         return fail(MRLBM_E_HIP, "hipStreamCreate failed");
+    // the stream-ordering events order work on this device only: a device-scope release
+    // (no system-scope cache writeback each time one is recorded)
+    const unsigned order_flags = hipEventDisableTiming | hipEventReleaseToDevice;
     if (hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess ||
-        hipEventCreateWithFlags(&h->evA, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&h->evB, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&h->evA, order_flags) != hipSuccess ||
+        hipEventCreateWithFlags(&h->evB, order_flags) != hipSuccess ||
+        hipEventCreateWithFlags(&h->evS, order_flags) != hipSuccess ||
+        hipEventCreateWithFlags(&h->evW, order_flags) != hipSuccess)
         return fail(MRLBM_E_HIP, "hipEventCreate failed");
     if (hipEventRecord(h->evA, h->st) != hipSuccess || hipEventRecord(h->evB, h->st2) != hipSuccess)
         return fail(MRLBM_E_HIP, "hipEventRecord failed");
@@ -355,8 +363,8 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
     }
     {
         // -1 auto: the strip height measured fastest at 8192 x 2048 on MI355X
-        const int tb = p->time_blocking < 0 ? (h->f32 ? 4 : 2) : p->time_blocking;
-        if (tb != 0 && tb != 2 && tb != 4 && tb != 8) return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4 or 8");
+        const int tb = p->time_blocking < 0 ? (h->f32 ? 10 : 8) : p->time_blocking;
+        if (tb < 0 || tb > 12 || tb % 2) return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4, 6, 8, 10 or 12");
         if (tb && h->nx >= 16 && h->ny >= 16) {
             const int rc3 = setup_time_blocking(h, tb);
             if (rc3) return fail(rc3, "");
@@ -608,16 +616,39 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         a.inlet_prev_c = h->prev_inlet_c;
         return a;
     };
-    // curved wall (or, without a cylinder, the inlet-density fold) of one iteration, after
-    // launch_side has done the ring, the bands and the wall's stencil nodes
-    auto side_wall = [&](const StepArgs<S, T> &a) -> hipError_t {
-        if (h->has_b) return launch_wall<S, T>(a, h->st2);
-        if (h->prev_inlet_valid) {
+    // Side work of one iteration on stream B: the ring and band kernel there, and in
+    // parallel on stream C the curved wall (stencil nodes, then the boundary workgroup) or,
+    // without a cylinder, the inlet-density fold; stream B then waits for stream C.
+    // Small grids are launch-bound: there the wall chain stays on stream B after the ring and
+    // band kernel (which then also does the stencil nodes), saving the fork and join.
+    const bool split_wall = (long long)h->nx * h->ny >= (1LL << 22);
+    auto side_work = [&](const StepArgs<S, T> &a) -> hipError_t {
+        if (!split_wall) {
+            hipError_t e = launch_side<S, T>(a, h->st2, true);
+            if (e != hipSuccess) return e;
+            if (h->has_b) return launch_wall<S, T>(a, h->st2);
+            if (h->prev_inlet_valid) {
+                auto f = a;
+                f.inlet_prev = h->d_inlet[h->prev_inlet_buf];
+                return launch_inlet_fold<S, T>(f, h->st2);
+            }
+            return hipSuccess;
+        }
+        hipError_t e = hipEventRecord(h->evS, h->st2);
+        if (e == hipSuccess) e = hipStreamWaitEvent(h->st3, h->evS, 0);
+        if (e == hipSuccess) e = launch_side<S, T>(a, h->st2, false);
+        if (e != hipSuccess) return e;
+        if (h->has_b) {
+            e = launch_stencil_moments<S, T>(a, h->st3);
+            if (e == hipSuccess) e = launch_wall<S, T>(a, h->st3);
+        } else if (h->prev_inlet_valid) {
             auto f = a;
             f.inlet_prev = h->d_inlet[h->prev_inlet_buf];
-            return launch_inlet_fold<S, T>(f, h->st2);
+            e = launch_inlet_fold<S, T>(f, h->st3);
         }
-        return hipSuccess;
+        if (e == hipSuccess) e = hipEventRecord(h->evW, h->st3);
+        if (e == hipSuccess) e = hipStreamWaitEvent(h->st2, h->evW, 0);
+        return e;
     };
     // inlet-density bookkeeping after iteration iter
     auto after_iter = [&](int iter) {
@@ -646,7 +677,29 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         const int nrec = (is_window(iter) ? 1 : 0) + (is_window(iter + 1) ? 1 : 0);
         return (int)rec_iters.size() + nrec <= std::max(h->rec_cap, 0) || nrec == 0;
     };
-    if (h->timing) HIPCHK(hipEventRecord(h->e0, h->st));
+    // timing mode: an event pair around every h->timing-th bulk launch (the first included),
+    // at most kMaxTimed launches
+    constexpr int kMaxTimed = 4096;
+    int ntimed = 0, nlaunch = 0;
+    if (h->timing) {
+        const size_t want = 2 * (size_t)std::min(nsteps / h->timing + 1, kMaxTimed);
+        while (h->tev.size() < want) {
+            hipEvent_t e; // timing only: read after e1 has synchronized with the work
+            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+            h->tev.push_back(e);
+        }
+        HIPCHK(hipEventRecord(h->e0, h->st));
+    }
+    auto mark = [&](int which) -> hipError_t {
+        if (!h->timing) return hipSuccess;
+        const bool timed = nlaunch % h->timing == 0 && ntimed < kMaxTimed;
+        hipError_t e = timed ? hipEventRecord(h->tev[2 * ntimed + which], h->st) : hipSuccess;
+        if (which == 1) {
+            ntimed += timed ? 1 : 0;
+            nlaunch++;
+        }
+        return e;
+    };
     for (int s = 0; s < nsteps; s++) {
         const int iter = iter_first + s;
         if (s + 1 < nsteps && pair_ok(iter)) {
@@ -659,18 +712,18 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             bulk.ncg = h->ncg2;
             bulk.ncb = h->ncb2;
             HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
+            HIPCHK(mark(0));
             HIPCHK(launch_bulk2<S, T>(bulk, h->ncb2 * h->nstrips2, h->tb_H, h->st));
+            HIPCHK(mark(1));
             HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
             a1.nrect = h->nrect1;
             std::memcpy(a1.rect, h->rect1, sizeof h->rect1);
-            HIPCHK(launch_side<S, T>(a1, h->st2));
-            HIPCHK(side_wall(a1));
+            HIPCHK(side_work(a1));
             after_iter(iter);
             auto a2 = step_args(iter + 1, h->S2, C, slot1 + (is_window(iter) ? 1 : 0));
             a2.nrect = h->nrect2;
             std::memcpy(a2.rect, h->rect2, sizeof h->rect2);
-            HIPCHK(launch_side<S, T>(a2, h->st2));
-            HIPCHK(side_wall(a2));
+            HIPCHK(side_work(a2));
             after_iter(iter + 1);
             HIPCHK(hipEventRecord(h->evB, h->st2));
             HIPCHK(hipEventRecord(h->evA, h->st));
@@ -686,10 +739,11 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         const int nb = (pops_step || a.simple || a.chi) ? simple_blocks : march_blocks;
         // bulk_n after wall_{n-1}; wall_n after bulk_{n-1}: both read state n-1 only
         HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
+        HIPCHK(mark(0));
         HIPCHK(launch_bulk<S, T>(a, nb, h->st));
+        HIPCHK(mark(1));
         HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
-        HIPCHK(launch_side<S, T>(a, h->st2));
-        HIPCHK(side_wall(a));
+        HIPCHK(side_work(a));
         HIPCHK(hipEventRecord(h->evB, h->st2));
         HIPCHK(hipEventRecord(h->evA, h->st));
         after_iter(iter);
@@ -711,6 +765,13 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, h->e0, h->e1));
         h->last_ms = nsteps > 0 ? ms / nsteps : 0.0;
+        double tot = 0;
+        for (int k = 0; k < ntimed; k++) {
+            HIPCHK(hipEventElapsedTime(&ms, h->tev[2 * k], h->tev[2 * k + 1]));
+            tot += ms;
+        }
+        h->last_bulk_n = ntimed;
+        h->last_bulk_ms = ntimed > 0 ? tot / ntimed : 0.0;
     }
     drain_records(h, rec_iters, opts.nprobe, opts.record_surface_p);
     h->last_nprobe = opts.nprobe;
@@ -907,7 +968,8 @@ extern "C" int mrlbm_check_finite(const mrlbm_handle *hc)
 extern "C" int mrlbm_set_timing(mrlbm_handle *h, int32_t on)
 {
     if (!h) return MRLBM_E_ARG;
-    h->timing = on != 0;
+    if (on < 0) FAIL(MRLBM_E_ARG, "timing must be 0 (off) or a launch stride >= 1");
+    h->timing = on;
     return 0;
 }
 
@@ -915,5 +977,13 @@ extern "C" int mrlbm_last_step_ms(const mrlbm_handle *h, double *ms)
 {
     if (!h || !ms) return MRLBM_E_ARG;
     *ms = h->last_ms;
+    return 0;
+}
+
+extern "C" int mrlbm_last_bulk_ms(const mrlbm_handle *h, double *ms_per_launch, int32_t *launches)
+{
+    if (!h || !ms_per_launch) return MRLBM_E_ARG;
+    *ms_per_launch = h->last_bulk_ms;
+    if (launches) *launches = h->last_bulk_n;
     return 0;
 }

@@ -357,6 +357,16 @@ template <class T> __device__ __forceinline__ T interp_weight(const double *R, i
 // Velocity at the boundary nodes from the stencil nodes' pre-collision moments U, level by
 // level so that a node reads the already-interpolated velocity of an earlier node, as the
 // in-place loop does (2d_mrlbm.f90:773-815).  One workgroup; writes b.ub.
+// Quadratic extrapolation to the wall from bilinear values at the second and third
+// reference points of boundary node R (2d_mrlbm.f90:783-815): c[0..3] the corners around
+// point 2, c[4..7] around point 3.
+template <class T> __device__ __forceinline__ T wall_interp(const double *R, const T c[8])
+{
+    const T xd2 = (T)R[BT_XD2], yd2 = (T)R[BT_YD2], xd3 = (T)R[BT_XD3], yd3 = (T)R[BT_YD3];
+    const T t1 = (T)R[BT_T1], t2 = (T)R[BT_T2], t3 = (T)R[BT_T3];
+    return (t1 * (T)0.0 + t2 * bilin(xd2, yd2, c[0], c[1], c[2], c[3])) - t3 * bilin(xd3, yd3, c[4], c[5], c[6], c[7]);
+}
+
 template <class S, class T> __device__ void wall_velocities(const StepArgs<S, T> &a)
 {
     const BoundaryDev &b = a.b;
@@ -375,14 +385,8 @@ template <class S, class T> __device__ void wall_velocities(const StepArgs<S, T>
                     cx[c] = dp >= 0 ? ub[2 * dp] : U[nU + u];
                     cy[c] = dp >= 0 ? ub[2 * dp + 1] : U[2 * nU + u];
                 }
-                const T xd2 = (T)R[BT_XD2], yd2 = (T)R[BT_YD2], xd3 = (T)R[BT_XD3], yd3 = (T)R[BT_YD3];
-                const T ux2 = bilin(xd2, yd2, cx[0], cx[1], cx[2], cx[3]);
-                const T uy2 = bilin(xd2, yd2, cy[0], cy[1], cy[2], cy[3]);
-                const T ux3 = bilin(xd3, yd3, cx[4], cx[5], cx[6], cx[7]);
-                const T uy3 = bilin(xd3, yd3, cy[4], cy[5], cy[6], cy[7]);
-                const T t1 = (T)R[BT_T1], t2 = (T)R[BT_T2], t3 = (T)R[BT_T3];
-                vx = (t1 * (T)0.0 + t2 * ux2) - t3 * ux3;
-                vy = (t1 * (T)0.0 + t2 * uy2) - t3 * uy3;
+                vx = wall_interp(R, cx);
+                vy = wall_interp(R, cy);
             } else {
                 vx = U[nU + b.uself[k]];
                 vy = U[2 * nU + b.uself[k]];
@@ -409,10 +413,8 @@ template <class T> __device__ __forceinline__ void wall_moments(const BoundaryDe
         cx[c] = U[3 * nU + u];
         cy[c] = U[4 * nU + u];
     }
-    const T xd2 = (T)R[BT_XD2], yd2 = (T)R[BT_YD2], xd3 = (T)R[BT_XD3], yd3 = (T)R[BT_YD3];
-    const T t1 = (T)R[BT_T1], t2 = (T)R[BT_T2], t3 = (T)R[BT_T3];
-    Mxx = (t1 * (T)0.0 + t2 * bilin(xd2, yd2, cx[0], cx[1], cx[2], cx[3])) - t3 * bilin(xd3, yd3, cx[4], cx[5], cx[6], cx[7]);
-    Myy = (t1 * (T)0.0 + t2 * bilin(xd2, yd2, cy[0], cy[1], cy[2], cy[3])) - t3 * bilin(xd3, yd3, cy[4], cy[5], cy[6], cy[7]);
+    Mxx = wall_interp(R, cx);
+    Myy = wall_interp(R, cy);
 }
 
 } // namespace mrlbm

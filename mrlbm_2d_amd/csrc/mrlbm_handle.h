@@ -36,11 +36,17 @@ struct mrlbm_handle {
     bool has_b = false;
     double nu = 0, omega = 0;
     hipStream_t st = nullptr;   // bulk kernel and every host-facing operation
-    hipStream_t st2 = nullptr;  // curved-wall kernels, concurrent with the bulk
-    hipEvent_t evA = nullptr, evB = nullptr; // end of the last bulk / wall kernels
+    hipStream_t st2 = nullptr;  // ring and band kernels, concurrent with the bulk
+    hipStream_t st3 = nullptr;  // curved-wall kernels, concurrent with the ring and band kernels
+    hipEvent_t evA = nullptr, evB = nullptr; // end of the last bulk / side work
+    hipEvent_t evS = nullptr, evW = nullptr; // fork to and join from the wall stream
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    bool timing = false;
+    int timing = 0; // 0 off, else time every timing-th bulk launch
     double last_ms = 0;
+    // timing mode: an event pair around each bulk launch on stream st (reused, grown on demand)
+    std::vector<hipEvent_t> tev;
+    double last_bulk_ms = 0;
+    int last_bulk_n = 0;
     void *S[2] = {nullptr, nullptr};
     int cur = 0;
     void *pops = nullptr;

@@ -81,10 +81,10 @@ hipError_t launch_stencil_moments(const StepArgs<S, T> &a, hipStream_t st);
 // from the dilated cylinder box), src = state t, dst = state t+2; strips of H rows.
 template <class S, class T>
 hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st);
-// Side stream of one step in two launches: ring + band rectangles + wall stencil nodes,
-// then the boundary workgroup.
+// Side work of one step: ring + band rectangles (+ the wall's stencil nodes when stencil),
+// and the boundary workgroup.
 template <class S, class T>
-hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st);
+hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st, bool stencil);
 template <class S, class T>
 hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st);
 // Plain node update over a.rect[0..nrect) (boundary nodes skipped): the band sub-steps.

@@ -14,6 +14,8 @@
 // edge_kernel on the side stream.
 #include "mrlbm_device.h"
 
+#include <algorithm>
+
 namespace mrlbm {
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b -> XCD b % 8), each XCD with
@@ -289,62 +291,15 @@ template <class S, class T, bool REC, int H> __device__ void bulk_march_a(const 
 
 // ---------------------------------------------------------------------------------
 // Two steps per pass (temporal blocking).  A wave owns 64 aligned columns x0..x0+63 and
-// marches up a strip; per row it first advances t -> t+1 for its own columns and for one
-// halo column per side (x0-1 in lane 0, x0+64 in lane 63), then t+1 -> t+2 for its own
-// columns with the single-step aligned row (march_row_a).  State t is loaded with one
-// extra per-lane load per plane: lanes 0, 1, 62, 63 fetch x0-2, x0-1, x0+64, x0+65 (the
-// other lanes repeat their own column, which coalesces).  Every node update is the
-// single-step arithmetic, so two passes of this kernel give the bits of four steps.
-
-// Level 1 of row r: pulls from rows r-1 (carried "up" sets), r ("mid" sets) and r+1 (F9,
-// H9: the freshly rebuilt own and halo populations), for the own node (f) and the halo
-// node (g); returns their post-collision moments and rolls the carries.
-template <class S, class T>
-__device__ __forceinline__ void level1_row(const StepArgs<S, T> &a, bool left, bool right, int nbl, const T F9[Q],
-                                           const T H9[Q], T up_prev[3], T up_cur[3], T mid_cur[3], T hup_prev[3],
-                                           T hup_cur[3], T hmid_cur[3], T m1[6], T m1h[6])
-{
-    using F = typename FormOf<T>::F;
-    T f[Q], g[Q];
-    // Every cross-lane read is done by all 64 lanes before any lane selects: a shuffle
-    // inside a branch would read the registers of lanes that branch left inactive.
-    const T s1 = shup(mid_cur[1]), s3 = shdn(mid_cur[2]), t1 = shdn(hmid_cur[1]), t3 = shup(hmid_cur[2]);
-    const T s5 = shup(up_prev[1]), s6 = shdn(up_prev[2]), t5 = shdn(hup_prev[1]), t6 = shup(hup_prev[2]);
-    const T s8 = shup(F9[8]), s7 = shdn(F9[7]), t8 = shdn(H9[8]), t7 = shup(H9[7]);
-    const T z0 = __shfl(hmid_cur[0], nbl, 64), z2 = __shfl(hup_prev[0], nbl, 64), z4 = __shfl(H9[4], nbl, 64);
-    // own node: x-shifted populations from the neighbour lanes, or for lanes 0 / 63 from
-    // the halo populations of lanes 1 / 62
-    f[0] = mid_cur[0];
-    f[1] = left ? t1 : s1;
-    f[3] = right ? t3 : s3;
-    f[2] = up_prev[0];
-    f[5] = left ? t5 : s5;
-    f[6] = right ? t6 : s6;
-    f[4] = F9[4];
-    f[8] = left ? t8 : s8;
-    f[7] = right ? t7 : s7;
-    // halo node (x0-1 in lane 0, x0+64 in lane 63): its own column is the halo of lane
-    // 1 / 62; the column beyond is this lane's halo, the column inside this lane's own
-    g[0] = z0;
-    g[2] = z2;
-    g[4] = z4;
-    g[1] = left ? hmid_cur[1] : mid_cur[1];
-    g[5] = left ? hup_prev[1] : up_prev[1];
-    g[8] = left ? H9[8] : F9[8];
-    g[3] = left ? mid_cur[2] : hmid_cur[2];
-    g[6] = left ? up_prev[2] : hup_prev[2];
-    g[7] = left ? F9[7] : H9[7];
-    F::moments(f, m1);
-    collide<T>(m1, a.om, a.om1, a.incomp_bulk);
-    F::moments(g, m1h);
-    collide<T>(m1h, a.om, a.om1, a.incomp_bulk);
-    up_prev[0] = up_cur[0]; up_prev[1] = up_cur[1]; up_prev[2] = up_cur[2];
-    up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
-    mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
-    hup_prev[0] = hup_cur[0]; hup_prev[1] = hup_cur[1]; hup_prev[2] = hup_cur[2];
-    hup_cur[0] = H9[2]; hup_cur[1] = H9[5]; hup_cur[2] = H9[6];
-    hmid_cur[0] = H9[0]; hmid_cur[1] = H9[1]; hmid_cur[2] = H9[3];
-}
+// marches up a strip of H output rows: per row it advances t -> t+1 for its own columns,
+// then t+1 -> t+2 for its own columns, with full-wave aligned stores.  What lanes 0 and 63
+// need from beyond the wave is computed once per strip, before the march, one lane per
+// value: pass 1 rebuilds the populations of the nodes in the three columns at each side
+// (x0-2..x0, x0+63..x0+65) over the strip's rows of state t; pass 2 advances the halo nodes
+// (x0-1, x0+64) of every t+1 row, gathering their pulled populations from pass 1.  The
+// march then takes the three populations that cross into the wave from those lanes with
+// one cross-lane read each.  Every node update is the single-step arithmetic, so two passes
+// of this kernel give the bits of four steps.
 
 // Output nodes of the two-step pass: two or more nodes from the outer ring and outside the
 // cylinder box dilated by one (everything else is finished by the band sub-steps).
@@ -354,9 +309,33 @@ template <class S, class T> __device__ __forceinline__ bool two_step_node(const 
     return !(a.bw > 0 && c >= a.bx0 - 1 && c <= a.bx0 + a.bw && j >= a.by0 - 1 && j <= a.by0 + a.bh);
 }
 
+// Pulled populations of a row from the carried rows below (up_prev: k = 2, 5, 6), the row
+// itself (mid: k = 0, 1, 3) and the row above (N9); x-shifts from the neighbour lanes, or in
+// lanes 0 / 63 the values crossing in from outside the wave (in0, in1, in2: the row's own,
+// the row below's and the row above's populations 1/3, 5/6, 8/7).
+template <class T>
+__device__ __forceinline__ void pull_row(bool left, bool right, const T up_prev[3], const T mid[3], const T N9[Q],
+                                         T in0, T in1, T in2, T f[Q])
+{
+    const T s1 = shup(mid[1]), s3 = shdn(mid[2]), s5 = shup(up_prev[1]), s6 = shdn(up_prev[2]);
+    const T s8 = shup(N9[8]), s7 = shdn(N9[7]);
+    f[0] = mid[0];
+    f[1] = left ? in0 : s1;
+    f[3] = right ? in0 : s3;
+    f[2] = up_prev[0];
+    f[5] = left ? in1 : s5;
+    f[6] = right ? in1 : s6;
+    f[4] = N9[4];
+    f[8] = left ? in2 : s8;
+    f[7] = right ? in2 : s7;
+}
+
 template <class S, class T, int H> __device__ void bulk_march2(const StepArgs<S, T> &a, int bb)
 {
     using F = typename FormOf<T>::F;
+    constexpr int NQ = H + 4;             // state-t rows j0-2 .. j0+H+1
+    constexpr int R1 = (6 * NQ + 63) / 64; // pass-1 rounds of 64 lanes
+    static_assert(R1 <= 2 && 2 * (H + 2) <= 64, "at most two pass-1 rounds, one pass-2 round");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int2 tile = bulk_tile(a, bb);
     const int cbi = tile.x, st = tile.y;
@@ -364,56 +343,111 @@ template <class S, class T, int H> __device__ void bulk_march2(const StepArgs<S,
     if (g >= a.ncg) return;
     const int x0 = g * 64, c = x0 + lane; // c < pitch: the pitch is a multiple of 64
     const bool left = lane == 0, right = lane == 63;
-    const int hx = lane == 0 ? x0 - 2 : (lane == 1 ? x0 - 1 : (lane == 62 ? x0 + 64 : (lane == 63 ? x0 + 65 : c)));
-    const int hc = min(max(hx, 0), a.pitch - 1);
-    const int nbl = left ? 1 : (right ? 62 : lane);
+    const int side = right ? 1 : 0;
     const int j0 = 2 + st * H; // output rows j0 .. j0+H-1 at t+2
-    S m[H + 4][6], h[H + 4][6];
+    // pass-1 node n: the halo columns first (n = side*NQ + q: column x0-1 / x0+64), then the
+    // outer and inner columns (n = 2NQ + side*2NQ + (col/2)*NQ + q, col 0 or 2); node n is
+    // rebuilt by lane n % 64 in round n / 64.  Its loads are issued before the strip's own,
+    // so waiting for them does not wait for the strip (loads complete in issue order for the
+    // wait counter).
+    auto node_of = [&](int n, int &x, int &q) {
+        const int s = n < 2 * NQ ? n / NQ : (n - 2 * NQ) / (2 * NQ);
+        int col = 1;
+        if (n < 2 * NQ) q = n - s * NQ;
+        else {
+            const int r = n - 2 * NQ - s * 2 * NQ;
+            col = r < NQ ? 0 : 2;
+            q = r < NQ ? r : r - NQ;
+        }
+        x = s ? x0 + 63 + col : x0 - 2 + col;
+    };
+    auto node_index = [&](int s, int col, int q) {
+        return col == 1 ? s * NQ + q : 2 * NQ + s * 2 * NQ + (col == 0 ? 0 : NQ) + q;
+    };
+    S pm[R1][6];
 #pragma unroll
-    for (int t = 0; t < H + 4; t++) { // state t rows j0-2 .. j0+H+1
-        const int r = min(j0 - 2 + t, a.ny - 1);
-        load_moments(a, c, r, m[t]);
-        load_moments(a, hc, r, h[t]);
+    for (int r = 0; r < R1; r++) {
+        int x, q;
+        node_of(min(r * 64 + lane, 6 * NQ - 1), x, q);
+        load_moments(a, min(max(x, 0), a.pitch - 1), min(max(j0 - 2 + q, 0), a.ny - 1), pm[r]);
     }
-    T F9[Q], H9[Q];
-    T up_prev[3], up_cur[3], mid_cur[3], hup_prev[3], hup_cur[3], hmid_cur[3];
+    S m[NQ][6];
+#pragma unroll
+    for (int t = 0; t < NQ; t++) load_moments(a, c, min(j0 - 2 + t, a.ny - 1), m[t]);
+    T P9[R1][Q];
+#pragma unroll
+    for (int r = 0; r < R1; r++) F::recon(pm[r], P9[r]);
+    // populations of the halo columns that move into the wave (round 0, lanes 0 .. 2NQ-1)
+    const bool s0 = lane >= NQ;
+    const T av0 = s0 ? P9[0][3] : P9[0][1], av1 = s0 ? P9[0][6] : P9[0][5], av2 = s0 ? P9[0][7] : P9[0][8];
+    // pass 2: lane = 2p + side advances the halo node of t+1 row j0-1+p (t row p+1)
+    T bv0, bv1, bv2;
+    {
+        const int s2 = lane & 1, p = lane >> 1;
+        T f[Q], mh[6], K9[Q];
+#pragma unroll
+        for (int k = 0; k < Q; k++) {
+            const int n = node_index(s2, 1 - EXk(k), min(p + 1 - EYk(k), NQ - 1));
+            const T v0 = __shfl(P9[0][k], n & 63, 64);
+            if constexpr (R1 > 1) {
+                const T v1 = __shfl(P9[R1 - 1][k], n & 63, 64);
+                f[k] = n < 64 ? v0 : v1;
+            } else {
+                f[k] = v0;
+            }
+        }
+        F::moments(f, mh);
+        collide<T>(mh, a.om, a.om1, a.incomp_bulk);
+        F::recon(mh, K9);
+        bv0 = s2 ? K9[3] : K9[1];
+        bv1 = s2 ? K9[6] : K9[5];
+        bv2 = s2 ? K9[7] : K9[8];
+    }
+    auto lane_a = [&](int q) { return side * NQ + q; };          // pass-1 lane of halo column, t row q
+    auto lane_b = [&](int p) { return 2 * p + side; };           // pass-2 lane of halo node, t+1 row p
+    T F9[Q], up_prev[3], up_cur[3], mid_cur[3];
     F::recon(m[0], F9);
-    F::recon(h[0], H9);
     up_prev[0] = F9[2]; up_prev[1] = F9[5]; up_prev[2] = F9[6];
-    hup_prev[0] = H9[2]; hup_prev[1] = H9[5]; hup_prev[2] = H9[6];
     F::recon(m[1], F9);
-    F::recon(h[1], H9);
     up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
     mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
-    hup_cur[0] = H9[2]; hup_cur[1] = H9[5]; hup_cur[2] = H9[6];
-    hmid_cur[0] = H9[0]; hmid_cur[1] = H9[1]; hmid_cur[2] = H9[3];
-    // level 1, rows j0-1 and j0: the first two t+1 rows seed the level-2 march
-    T m1[6], m1h[6];
-    T G9[Q], K9[Q], vup_prev[3], vup_cur[3], vmid_cur[3], ku_prev[1], ku_cur[1], km_cur[1];
-    F::recon(m[2], F9);
-    F::recon(h[2], H9);
-    level1_row(a, left, right, nbl, F9, H9, up_prev, up_cur, mid_cur, hup_prev, hup_cur, hmid_cur, m1, m1h);
-    F::recon(m1, G9);
-    F::recon(m1h, K9);
-    vup_prev[0] = G9[2]; vup_prev[1] = G9[5]; vup_prev[2] = G9[6];
-    ku_prev[0] = left ? K9[5] : K9[6];
-    F::recon(m[3], F9);
-    F::recon(h[3], H9);
-    level1_row(a, left, right, nbl, F9, H9, up_prev, up_cur, mid_cur, hup_prev, hup_cur, hmid_cur, m1, m1h);
-    F::recon(m1, G9);
-    F::recon(m1h, K9);
-    vup_cur[0] = G9[2]; vup_cur[1] = G9[5]; vup_cur[2] = G9[6];
-    vmid_cur[0] = G9[0]; vmid_cur[1] = G9[1]; vmid_cur[2] = G9[3];
-    ku_cur[0] = left ? K9[5] : K9[6];
-    km_cur[0] = left ? K9[1] : K9[3];
+    T vup_prev[3], vup_cur[3], vmid_cur[3];
 #pragma unroll
-    for (int t = 0; t < H; t++) {
-        // level 1 of row j0+1+t, then level 2 of row j0+t
-        F::recon(m[t + 4], F9);
-        F::recon(h[t + 4], H9);
-        level1_row(a, left, right, nbl, F9, H9, up_prev, up_cur, mid_cur, hup_prev, hup_cur, hmid_cur, m1, m1h);
-        march_row_a<S, T, false>(a, c, two_step_node(a, c, j0 + t), j0 + t, m1, m1h, left, right, vup_prev, vup_cur,
-                                 vmid_cur, ku_prev, ku_cur, km_cur, 0);
+    for (int p = 0; p < H + 2; p++) {
+        // level 1: t+1 row j0-1+p from t rows p, p+1, p+2
+        F::recon(m[p + 2], F9);
+        const T in0 = __shfl(av0, lane_a(p + 1), 64), in1 = __shfl(av1, lane_a(p), 64), in2 = __shfl(av2, lane_a(p + 2), 64);
+        T f[Q], m1[6], G9[Q];
+        pull_row(left, right, up_prev, mid_cur, F9, in0, in1, in2, f);
+        F::moments(f, m1);
+        collide<T>(m1, a.om, a.om1, a.incomp_bulk);
+        up_prev[0] = up_cur[0]; up_prev[1] = up_cur[1]; up_prev[2] = up_cur[2];
+        up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
+        mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
+        F::recon(m1, G9);
+        if (p >= 2) {
+            // level 2: t+2 row j0+p-2 from t+1 rows p-2, p-1, p
+            const T i0 = __shfl(bv0, lane_b(p - 1), 64), i1 = __shfl(bv1, lane_b(p - 2), 64), i2 = __shfl(bv2, lane_b(p), 64);
+            T g2[Q], m2[6];
+            pull_row(left, right, vup_prev, vmid_cur, G9, i0, i1, i2, g2);
+            const int j = j0 + p - 2;
+            if (two_step_node(a, c, j)) {
+                F::moments(g2, m2);
+                collide<T>(m2, a.om, a.om1, a.incomp_bulk);
+                const long long n = (long long)j * a.pitch + c;
+#pragma unroll
+                for (int q = 0; q < 6; q++) a.dst[q * a.plane + n] = (S)m2[q];
+            }
+        }
+        if (p >= 1) {
+            vup_prev[0] = p == 1 ? vup_prev[0] : vup_cur[0];
+            vup_prev[1] = p == 1 ? vup_prev[1] : vup_cur[1];
+            vup_prev[2] = p == 1 ? vup_prev[2] : vup_cur[2];
+            vup_cur[0] = G9[2]; vup_cur[1] = G9[5]; vup_cur[2] = G9[6];
+            vmid_cur[0] = G9[0]; vmid_cur[1] = G9[1]; vmid_cur[2] = G9[3];
+        } else {
+            vup_prev[0] = G9[2]; vup_prev[1] = G9[5]; vup_prev[2] = G9[6];
+        }
     }
 }
 
@@ -540,96 +574,130 @@ template <class S, class T> __global__ void __launch_bounds__(256) stencil_momen
     stencil_moments_body(a, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Wall BC, collision, store, momentum exchange and surface-pressure feed of the boundary
-// nodes (one workgroup), after stencil_moments_kernel of the same step.
-template <class S, class T> __device__ void boundary_body(const StepArgs<S, T> &a)
+// Wall BC, collision, store, momentum exchange and surface-pressure feed of boundary node k,
+// after stencil_moments_kernel of the same step.  ONE_PASS (one dependency level: no
+// boundary velocity is interpolated from another boundary node's): the node interpolates its
+// own wall velocity, its stencil loads issued together with the gather so that the thread
+// waits for one memory round trip; otherwise ub holds the velocities from wall_velocities.
+template <class S, class T, bool ONE_PASS> __device__ __forceinline__ void wall_node(const StepArgs<S, T> &a, int k)
 {
     using F = typename FormOf<T>::F;
     const BoundaryDev &b = a.b;
-    __shared__ double sh[16];
     T *U = (T *)b.U, *ub = (T *)b.ub;
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nU = b.nU;
     const T om = a.om;
-    wall_velocities(a); // 2. velocity at the boundary nodes
-    // 3. wall BC, collision, store, forces, surface pressure
-    for (int k = tid; k < b.nbct; k += nt) {
-        const int i = b.ij[2 * k], j = b.ij[2 * k + 1];
-        const double *R = b.real + (long long)k * BT_NREAL;
-        T fp[Q], ff[Q];
-        gather_interior(a, i, j, fp);
+    const bool interp_u = b.vel_interp || !b.rotated;
+    const int i = b.ij[2 * k], j = b.ij[2 * k + 1];
+    const double *R = b.real + (long long)k * BT_NREAL;
+    T cu[8], cv[8], cm[8], cn[8], su = (T)0.0, sv = (T)0.0;
+    if constexpr (ONE_PASS) { // stencil u, v, Mxx', Myy' (every uc index is a stencil node)
 #pragma unroll
-        for (int q = 0; q < Q; q++) ff[q] = F::dev ? fp[q] + F::base(q) : fp[q];
-        const T ux = ub[2 * k], uy = ub[2 * k + 1];
-        const int lab = b.label[k];
-        const uint16_t in = b.incs[lab], out = b.outs[lab];
-        T m[6];
-        if (b.rotated) {
-            T Mxx, Myy;
+        for (int c = 0; c < 8; c++) {
+            const int u = b.uc[12 * k + c];
+            cu[c] = U[nU + u];
+            cv[c] = U[2 * nU + u];
+            cm[c] = U[3 * nU + u];
+            cn[c] = U[4 * nU + u];
+        }
+        const int u = b.uself[k];
+        su = U[nU + u];
+        sv = U[2 * nU + u];
+    }
+    T fp[Q], ff[Q];
+    gather_interior(a, i, j, fp);
+#pragma unroll
+    for (int q = 0; q < Q; q++) ff[q] = F::dev ? fp[q] + F::base(q) : fp[q];
+    T ux, uy;
+    if constexpr (ONE_PASS) {
+        ux = interp_u ? wall_interp(R, cu) : su;
+        uy = interp_u ? wall_interp(R, cv) : sv;
+        ub[2 * k] = ux;
+        ub[2 * k + 1] = uy;
+    } else {
+        ux = ub[2 * k];
+        uy = ub[2 * k + 1];
+    }
+    const int lab = b.label[k];
+    const uint16_t in = b.incs[lab], out = b.outs[lab];
+    T m[6];
+    if (b.rotated) {
+        T Mxx = (T)0.0, Myy = (T)0.0;
+        if constexpr (!ONE_PASS) {
             wall_moments(b, U, k, Mxx, Myy);
-            const T c = (T)R[BT_CTH], s = (T)R[BT_STH];
-            const T uxp = ux * c + uy * s, uyp = uy * c - ux * s; // :304-305
-            wall_bc_rotated<T>(ff, uxp, uyp, R, in, out, b.mass_bc, om, Mxx, Myy, m);
-        } else {
-            wall_bc_unrotated<T>(ff, ux, uy, in, out, om, m);
+        } else if (b.mom_interp) {
+            Mxx = wall_interp(R, cm);
+            Myy = wall_interp(R, cn);
         }
-        m[1] = ux;
-        m[2] = uy;
-        const long long n = (long long)j * a.pitch + i;
-        if (a.rec_slot >= 0)
-            for (int p = 0; p < a.nprobe; p++)
-                if (n == a.probe_node[p]) {
-                    double *r = a.rec + (long long)a.rec_slot * a.rec_stride + 2 + 3 * p;
-                    r[0] = m[0];
-                    r[1] = m[1];
-                    r[2] = m[2];
-                }
-        if (a.sample) {
-            const double vx = m[1], vy = m[2];
-            a.acc[n] += vx;
-            a.acc[a.plane + n] += vy;
-            a.acc[2 * a.plane + n] += vx * vx;
-            a.acc[3 * a.plane + n] += vy * vy;
-            a.acc[4 * a.plane + n] += vx * vy;
+        const T c = (T)R[BT_CTH], s = (T)R[BT_STH];
+        const T uxp = ux * c + uy * s, uyp = uy * c - ux * s; // :304-305
+        wall_bc_rotated<T>(ff, uxp, uyp, R, in, out, b.mass_bc, om, Mxx, Myy, m);
+    } else {
+        wall_bc_unrotated<T>(ff, ux, uy, in, out, om, m);
+    }
+    m[1] = ux;
+    m[2] = uy;
+    const long long n = (long long)j * a.pitch + i;
+    if (a.rec_slot >= 0)
+        for (int p = 0; p < a.nprobe; p++)
+            if (n == a.probe_node[p]) {
+                double *r = a.rec + (long long)a.rec_slot * a.rec_stride + 2 + 3 * p;
+                r[0] = m[0];
+                r[1] = m[1];
+                r[2] = m[2];
+            }
+    if (a.sample) {
+        const double vx = m[1], vy = m[2];
+        a.acc[n] += vx;
+        a.acc[a.plane + n] += vy;
+        a.acc[2 * a.plane + n] += vx * vx;
+        a.acc[3 * a.plane + n] += vy * vy;
+        a.acc[4 * a.plane + n] += vx * vy;
+    }
+    collide<T>(m, om, a.om1, a.incomp_bulk);
+    S ms[6];
+    ms[0] = (S)F::store_rho(m[0]);
+#pragma unroll
+    for (int p = 1; p < 6; p++) ms[p] = (S)m[p];
+#pragma unroll
+    for (int p = 0; p < 6; p++) a.dst[p * a.plane + n] = ms[p];
+    if (a.rec_slot >= 0) { // momentum exchange (:558-591)
+        T fo[Q];
+        F::recon(ms, fo);
+        double fxi = 0, fyi = 0, fxo = 0, fyo = 0;
+        for (int q = 0; q < Q; q++) {
+            if ((out >> q) & 1) {
+                fxo = fxo + (double)fo[q] * (double)EXk(q);
+                fyo = fyo + (double)fo[q] * (double)EYk(q);
+            }
+            if ((in >> q) & 1) {
+                fxi = fxi + (double)fp[q] * (double)EXk(q);
+                fyi = fyi + (double)fp[q] * (double)EYk(q);
+            }
         }
-        collide<T>(m, om, a.om1, a.incomp_bulk);
-        S ms[6];
-        ms[0] = (S)F::store_rho(m[0]);
-#pragma unroll
-        for (int p = 1; p < 6; p++) ms[p] = (S)m[p];
-#pragma unroll
-        for (int p = 0; p < 6; p++) a.dst[p * a.plane + n] = ms[p];
-        if (a.rec_slot >= 0) { // momentum exchange (:558-591)
-            T fo[Q];
-            F::recon(ms, fo);
-            double fxi = 0, fyi = 0, fxo = 0, fyo = 0;
-            for (int q = 0; q < Q; q++) {
-                if ((out >> q) & 1) {
-                    fxo = fxo + (double)fo[q] * (double)EXk(q);
-                    fyo = fyo + (double)fo[q] * (double)EYk(q);
+        b.fxy[2 * k] = (fxi - fxo) + (F::dev ? b.wfx[lab] : 0.0);
+        b.fxy[2 * k + 1] = (fyi - fyo) + (F::dev ? b.wfy[lab] : 0.0);
+        if (a.record_p) { // extrapolated surface pressure (:620-676), p = rho*cs2 pre-BC
+            double pv[3];
+            for (int r = 0; r < 3; r++) {
+                double pc[4];
+                for (int c = 0; c < 4; c++) {
+                    const double s0 = (double)U[b.uc[12 * k + 4 * r + c]];
+                    pc[c] = (F::dev ? 1.0 + s0 : s0) * (1.0 / 3.0);
                 }
-                if ((in >> q) & 1) {
-                    fxi = fxi + (double)fp[q] * (double)EXk(q);
-                    fyi = fyi + (double)fp[q] * (double)EYk(q);
-                }
+                pv[r] = bilin<double>(R[BT_XD2 + 2 * r], R[BT_YD2 + 2 * r], pc[0], pc[1], pc[2], pc[3]);
             }
-            b.fxy[2 * k] = (fxi - fxo) + (F::dev ? b.wfx[lab] : 0.0);
-            b.fxy[2 * k + 1] = (fyi - fyo) + (F::dev ? b.wfy[lab] : 0.0);
-            if (a.record_p) { // extrapolated surface pressure (:620-676), p = rho*cs2 pre-BC
-                double pv[3];
-                for (int r = 0; r < 3; r++) {
-                    double pc[4];
-                    for (int c = 0; c < 4; c++) {
-                        const double s0 = (double)U[b.uc[12 * k + 4 * r + c]];
-                        pc[c] = (F::dev ? 1.0 + s0 : s0) * (1.0 / 3.0);
-                    }
-                    pv[r] = bilin<double>(R[BT_XD2 + 2 * r], R[BT_YD2 + 2 * r], pc[0], pc[1], pc[2], pc[3]);
-                }
-                b.pcy[k] = quadratic_fit(R[BT_R2], pv[0], R[BT_R3], pv[1], R[BT_R4], pv[2], R[BT_RS]);
-            }
+            b.pcy[k] = quadratic_fit(R[BT_R2], pv[0], R[BT_R3], pv[1], R[BT_R4], pv[2], R[BT_RS]);
         }
     }
-    __syncthreads();
-    // 4. reductions and records
+}
+
+// Net force, records and the inlet-density running mean of one step (one workgroup), after
+// every boundary node of the step has been written.
+template <class S, class T> __device__ void wall_records(const StepArgs<S, T> &a)
+{
+    const BoundaryDev &b = a.b;
+    __shared__ double sh[16];
+    const int tid = threadIdx.x, nt = blockDim.x;
     if (a.rec_slot >= 0) {
         double sx = 0.0, sy = 0.0;
         for (int k = tid; k < b.nbct; k += nt) {
@@ -678,9 +746,39 @@ template <class S, class T> __device__ void boundary_body(const StepArgs<S, T> &
     }
 }
 
-template <class S, class T> __global__ void __launch_bounds__(256) boundary_kernel(StepArgs<S, T> a)
+// All boundary nodes of one step in one workgroup: velocities (by dependency level), nodes,
+// records.
+template <class S, class T> __device__ void boundary_body(const StepArgs<S, T> &a)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (a.b.nlevels <= 1) {
+        for (int k = tid; k < a.b.nbct; k += nt) wall_node<S, T, true>(a, k);
+    } else {
+        wall_velocities(a);
+        for (int k = tid; k < a.b.nbct; k += nt) wall_node<S, T, false>(a, k);
+    }
+    __syncthreads();
+    wall_records(a);
+}
+
+// One workgroup of up to 512 threads (one boundary node per thread up to nbct = 512).
+template <class S, class T> __global__ void __launch_bounds__(512) boundary_kernel(StepArgs<S, T> a)
 {
     boundary_body(a);
+}
+
+// One boundary node per thread in single-wave workgroups (one dependency level only): a
+// workgroup this small finds room on a CU beside a running bulk launch as soon as one of the
+// bulk's waves retires.
+template <class S, class T> __global__ void __launch_bounds__(64) wall_nodes_kernel(StepArgs<S, T> a)
+{
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k < a.b.nbct) wall_node<S, T, true>(a, k);
+}
+
+template <class S, class T> __global__ void __launch_bounds__(256) wall_records_kernel(StepArgs<S, T> a)
+{
+    wall_records(a);
 }
 
 // H: 0 row loop over a runtime strip height, 4/8/16 unrolled strip of H rows (62 written
@@ -879,7 +977,9 @@ hipError_t launch_bulk2_batch(const StepArgs<S, T> *args, const int2 *tiles, int
     switch (H) {
     case 2: hipLaunchKernelGGL((bulk2_batch_kernel<S, T, 2>), g, b, 0, st, args, tiles, ntiles); break;
     case 4: hipLaunchKernelGGL((bulk2_batch_kernel<S, T, 4>), g, b, 0, st, args, tiles, ntiles); break;
+    case 6: hipLaunchKernelGGL((bulk2_batch_kernel<S, T, 6>), g, b, 0, st, args, tiles, ntiles); break;
     case 8: hipLaunchKernelGGL((bulk2_batch_kernel<S, T, 8>), g, b, 0, st, args, tiles, ntiles); break;
+    case 10: hipLaunchKernelGGL((bulk2_batch_kernel<S, T, 10>), g, b, 0, st, args, tiles, ntiles); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -890,7 +990,10 @@ template <class S, class T> hipError_t launch_bulk2(const StepArgs<S, T> &a, int
     switch (H) {
     case 2: hipLaunchKernelGGL((bulk2_kernel<S, T, 2>), g, b, 0, st, a); break;
     case 4: hipLaunchKernelGGL((bulk2_kernel<S, T, 4>), g, b, 0, st, a); break;
+    case 6: hipLaunchKernelGGL((bulk2_kernel<S, T, 6>), g, b, 0, st, a); break;
     case 8: hipLaunchKernelGGL((bulk2_kernel<S, T, 8>), g, b, 0, st, a); break;
+    case 10: hipLaunchKernelGGL((bulk2_kernel<S, T, 10>), g, b, 0, st, a); break;
+    case 12: hipLaunchKernelGGL((bulk2_kernel<S, T, 12>), g, b, 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -902,23 +1005,32 @@ template <class S, class T> hipError_t launch_band(const StepArgs<S, T> &a, hipS
     if (n > 0) hipLaunchKernelGGL((band_kernel<S, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
-template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st)
+template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st, bool stencil)
 {
     const int nring = 2 * a.nx + 2 * (a.ny - 2);
     long long nband = 0;
     for (int q = 0; q < a.nrect; q++) nband += (long long)a.rect[q][2] * a.rect[q][3];
-    const long long n = nring + nband + (a.has_boundary ? a.b.nU : 0);
-    hipLaunchKernelGGL((side_kernel<S, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, nring, (int)nband);
+    const long long n = nring + nband + (stencil && a.has_boundary ? a.b.nU : 0);
+    // single-wave workgroups: they fit beside a running bulk launch (see wall_nodes_kernel)
+    hipLaunchKernelGGL((side_kernel<S, T>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, a, nring, (int)nband);
     return hipGetLastError();
 }
 template <class S, class T> hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st)
 {
-    hipLaunchKernelGGL((boundary_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
+    if (a.b.nlevels <= 1) {
+        if (a.b.nbct > 0)
+            hipLaunchKernelGGL((wall_nodes_kernel<S, T>), dim3((a.b.nbct + 63) / 64), dim3(64), 0, st, a);
+        if (a.rec_slot >= 0 || a.inlet_prev)
+            hipLaunchKernelGGL((wall_records_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
+        return hipGetLastError();
+    }
+    const int threads = std::min(512, std::max(256, (a.b.nbct + 63) / 64 * 64));
+    hipLaunchKernelGGL((boundary_kernel<S, T>), dim3(1), dim3(threads), 0, st, a);
     return hipGetLastError();
 }
 template <class S, class T> hipError_t launch_stencil_moments(const StepArgs<S, T> &a, hipStream_t st)
 {
-    hipLaunchKernelGGL((stencil_moments_kernel<S, T>), dim3((a.b.nU + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((stencil_moments_kernel<S, T>), dim3((a.b.nU + 63) / 64), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 template <class S, class T> hipError_t launch_inlet_fold(const StepArgs<S, T> &a, hipStream_t st)
@@ -952,7 +1064,7 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_stencil_moments<S, T>(const StepArgs<S, T> &, hipStream_t);       \
     template hipError_t launch_bulk2<S, T>(const StepArgs<S, T> &, int, int, hipStream_t);       \
     template hipError_t launch_band<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
-    template hipError_t launch_side<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
+    template hipError_t launch_side<S, T>(const StepArgs<S, T> &, hipStream_t, bool);            \
     template hipError_t launch_wall<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
     template hipError_t launch_bulk_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, int, hipStream_t); \
     template hipError_t launch_side_batch<S, T>(const StepArgs<S, T> *, int, int, int, hipStream_t); \

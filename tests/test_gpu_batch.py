@@ -38,7 +38,7 @@ def single(p, run, n1, n2):
 
 @pytest.mark.parametrize("precision", [capi.MRLBM_FP32_DEV, capi.MRLBM_FP64])
 @pytest.mark.parametrize("kernel,persistent,time_blocking", [(0, -1, -1), (8, 0, 0), (304, 1, 0), (2, 1, 0),
-                                                             (304, 0, 4), (8, 0, 2), (16, 1, 8)])
+                                                             (304, 0, 4), (8, 0, 2), (16, 1, 6)])
 def test_batch_matches_single_runs(precision, kernel, persistent, time_blocking):
     ps = runs(precision)
     o = capi.default_batch_opts()

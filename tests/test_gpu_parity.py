@@ -25,7 +25,8 @@ def run_golden(name, kernel=0, time_blocking=0):
     if "f0" in g:
         lat.set_populations(g["f0"])
     n = int(g["nsteps"])
-    print(f"[gpu] golden {name} {lat.nx}x{lat.ny} steps={n} kernel={kernel} time_blocking={time_blocking}", flush=True)
+    print(f"[gpu] golden {name} {lat.nx}x{lat.ny} steps={n} kernel={kernel} time_blocking={time_blocking} "
+          f"wall levels={lat.info.levels}", flush=True)
     opts = step_opts_from_namelist(p)
     # stop at iter n, where the reference last appended to rho_mean_inlet.dat (every 100 steps)
     lat.step(2, n - 1, opts)
@@ -77,7 +78,7 @@ def test_bulk_kernel_variants_agree(name, kernel):
         np.testing.assert_array_equal(uy, g["uy"])
 
 
-@pytest.mark.parametrize("time_blocking", [2, 8])
+@pytest.mark.parametrize("time_blocking", [2, 6, 8, 12])
 @pytest.mark.parametrize("name", ["p1_walls", "s_periodic", "s_xyper_noise", "chan"])
 def test_time_blocking_strips_agree(name, time_blocking):
     g, lat = run_golden(name, time_blocking=time_blocking)
@@ -96,7 +97,7 @@ def test_fp32_variants_identical_and_close_to_reference(name):
     g = gc.load(name)
     p = g["params"]
     results = {}
-    for kernel in (0, 1, 2, 4, 8, 16, 304, 308, 316, 1002, 1004, 1008):
+    for kernel in (0, 1, 2, 4, 8, 16, 304, 308, 316, 1002, 1004, 1006, 1010, 1012):
         par = params_from_namelist(p, precision=capi.MRLBM_FP32_DEV)
         par.kernel = kernel if kernel < 1000 else 0
         par.time_blocking = kernel - 1000 if kernel >= 1000 else 0
@@ -122,3 +123,69 @@ def test_fp32_variants_identical_and_close_to_reference(name):
     assert drho < 1e-5
     assert dux < max(5e-5 * umax / 0.05, 5e-7)
     assert duy < max(5e-5 * umax / 0.05, 5e-7)
+
+
+@pytest.mark.parametrize("time_blocking,every,launches", [(0, 1, 10), (4, 1, 5), (0, 4, 3), (4, 4, 2)])
+def test_bulk_timing_counts_launches(time_blocking, every, launches):
+    """Timing mode records an event pair around every n-th bulk launch (a two-step launch
+    covers two steps; 10 steps = 10 or 5 launches) and leaves the result unchanged."""
+    g = gc.load("p1_walls")
+    out = []
+    for timing in (0, every):
+        par = params_from_namelist(g["params"])
+        par.time_blocking = time_blocking
+        with Lattice(par) as lat:
+            print(f"[gpu] timing={timing} time_blocking={time_blocking}", flush=True)
+            lat.set_timing(timing)
+            lat.step(2, 10)
+            ms, n = lat.last_bulk_ms()
+            if timing:
+                assert n == launches
+                assert 0 < ms < 1e3
+                assert lat.last_step_ms() > 0
+            else:
+                assert n == 0 and ms == 0
+            out.append(lat.fields())
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_large_grid_matches_oracle(oracle_mod):
+    """From 4M nodes up the curved wall runs on a stream of its own beside the ring and band
+    kernel (below that both share one stream).  FP64 at 4352 x 1088 (the reference's
+    L_front/back/top/bot = 8/55/8/7 with ncy = 68) against the oracle, from the rest state
+    with seeded +-1e-3 relative noise on every population: a single step from populations,
+    two two-step passes, a single step; fields bit for bit and the drag/lift series."""
+    p = dict(gc.load("p1_walls")["params"])
+    p.update(ncy=68, L_front=8, L_back=55, L_top=8, L_bot=7, Re=200.0, uo=0.05, nx=0, ny=0,
+             iplot=100000, statsbegin=2)
+    o = oracle_mod.Oracle(gc.oracle_params(oracle_mod, p))
+    assert o.nx * o.ny >= 1 << 22
+    par = params_from_namelist(p)
+    assert (par.nx, par.ny) == (o.nx, o.ny)
+    f0 = o.f() * (1.0 + np.random.default_rng(11).uniform(-1e-3, 1e-3, (o.nx, o.ny, 9)))
+    f0 = np.asfortranarray(f0)
+    o.set_state(f0)
+    with Lattice(par) as lat:
+        print(f"[gpu] large grid {lat.nx}x{lat.ny} vs oracle, wall levels={lat.info.levels}", flush=True)
+        lat.set_populations(f0)
+        lat.step(2, 6, step_opts_from_namelist(p))
+        fd, fl = [], []
+        for it in range(2, 8):
+            o.step(it)
+            a, b = o.forces()
+            fd.append(a)
+            fl.append(b)
+        rho, ux, uy = lat.fields()
+        orho, oux, ouy, *_ = o.moments()
+        np.testing.assert_array_equal(rho, orho)
+        np.testing.assert_array_equal(ux, oux)
+        np.testing.assert_array_equal(uy, ouy)
+        it, gfd, gfl = lat.forces()
+        np.testing.assert_array_equal(it, np.arange(2, 8, dtype=np.int32))
+        # The net force sums nbct node forces of size up to ~1/3 (the wall pressure) that
+        # nearly cancel here; another summation order moves it by ~eps * sum|node force| per
+        # add, so the absolute bound is 1e-15 per wall node.
+        atol = 1e-15 * o.nbct
+        np.testing.assert_allclose(gfd, fd, rtol=1e-12, atol=atol)
+        np.testing.assert_allclose(gfl, fl, rtol=1e-12, atol=atol)
