@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # steps per launch: FP32 strips of 10 rows, FP64 of 8), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1);
 # the summary and the commands are in profiles/bulk2_kernel_8192x2048_hbm_bytes.md
-TRAFFIC_BYTES = {"f32": 947_831_915, "f64": 1_797_906_848}
+TRAFFIC_BYTES = {"f32": 947_798_552, "f64": 1_797_846_091}
 NX, NY = 8192, 2048
 # HIP event pairs around every 4th bulk launch: the per-launch mean without the events' own
 # cost (about 2% of the step when recorded around every launch)
@@ -45,7 +45,7 @@ XC, YC, RC = 1024.0, 1025.0, 64.0
 UMAX, RE = 0.05, 200.0
 
 
-def make_lattice(precision, nx=NX, ny=NY, strip_rows=0, kernel=0):
+def make_lattice(precision, nx=NX, ny=NY, strip_rows=0, kernel=0, time_blocking=-1):
     from mrlbm_2d_amd import capi
     p = capi.default_params()
     p.nx, p.ny = nx, ny
@@ -57,6 +57,7 @@ def make_lattice(precision, nx=NX, ny=NY, strip_rows=0, kernel=0):
     p.precision = precision
     p.strip_rows = strip_rows
     p.kernel = kernel
+    p.time_blocking = time_blocking
     lat = capi.Lattice(p)
     rng = np.random.default_rng(3)
     ux = rng.uniform(-1e-4, 1e-4, (nx, ny))
@@ -230,6 +231,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--strip-rows", type=int, default=0)
+    ap.add_argument("--time-blocking", type=int, default=-1,
+                    help="-1 auto (two steps per bulk pass), 0 one step per pass, 2..12 two-step strip rows")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (for profiler counter scaling)")
     args = ap.parse_args()
@@ -243,10 +246,11 @@ def main():
     from mrlbm_2d_amd import capi
     print(f"[bench] FP32 deviation form {NX}x{NY}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
-    r32 = time_form(capi.MRLBM_FP32_DEV, args.steps, args.warmup, strip_rows=args.strip_rows)
+    kw = dict(strip_rows=args.strip_rows, time_blocking=args.time_blocking)
+    r32 = time_form(capi.MRLBM_FP32_DEV, args.steps, args.warmup, **kw)
     print(f"[bench] FP32: {r32}", file=sys.stderr, flush=True)
     print(f"[bench] FP64 plain form {NX}x{NY}", file=sys.stderr, flush=True)
-    r64 = time_form(capi.MRLBM_FP64, args.steps, args.warmup, strip_rows=args.strip_rows)
+    r64 = time_form(capi.MRLBM_FP64, args.steps, args.warmup, **kw)
     print(f"[bench] FP64: {r64}", file=sys.stderr, flush=True)
     adj = None
     if not args.no_adjoint:
@@ -261,7 +265,14 @@ def main():
         print("[bench] CPU baseline: oracle", file=sys.stderr, flush=True)
         cpu = cpu_baseline()
         print(f"[bench] CPU: {cpu}", file=sys.stderr, flush=True)
-    default_shape = args.strip_rows == 0
+    default_shape = args.strip_rows == 0 and args.time_blocking == -1
+    tb = args.time_blocking
+    if tb == -1:
+        k32, k64 = "bulk2_kernel<float,float,10>", "bulk2_kernel<double,double,8>"
+    elif tb == 0:
+        k32, k64 = "bulk_kernel<float,float,false,304>", "bulk_kernel<double,double,false,8>"
+    else:
+        k32, k64 = f"bulk2_kernel<float,float,{tb}>", f"bulk2_kernel<double,double,{tb}>"
     out = {
         "metric": "MLUPS of the D2Q9 regularized LBM step, FP32 deviation form (FP64 beside it)",
         "value": round(r32["mlups"], 1),
@@ -277,10 +288,9 @@ def main():
         "data": "synthetic (seeded +-1e-4 velocity noise)",
         "config": {"workload": "8192x2048 channel, cylinder D=128 at (1024,1025), "
                                "parabolic inlet u_max=0.05, Re=200", "nx": NX, "ny": NY},
-        "roofline": roofline(r32, "bulk2_kernel<float,float,10>", TRAFFIC_BYTES["f32"] if default_shape else None),
+        "roofline": roofline(r32, k32, TRAFFIC_BYTES["f32"] if default_shape else None),
         "fp64": {"value": round(r64["mlups"], 1), "ms_per_step": round(r64["ms_per_step"], 5),
-                 "roofline": roofline(r64, "bulk2_kernel<double,double,8>",
-                                      TRAFFIC_BYTES["f64"] if default_shape else None)},
+                 "roofline": roofline(r64, k64, TRAFFIC_BYTES["f64"] if default_shape else None)},
         "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3),
         "adjoint": adj,
         "batched": batch,

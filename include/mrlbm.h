@@ -244,15 +244,15 @@ int mrlbm_batch_forces(mrlbm_batch *b, double *fd_sum, double *fl_sum, double *f
 int mrlbm_batch_reset_forces(mrlbm_batch *b);
 int mrlbm_batch_get_info(const mrlbm_batch *b, mrlbm_batch_info *info);
 
-/* Timing mode: on = 0 off; on = n >= 1 records a HIP event pair around every n-th bulk launch
- * of each mrlbm_step call (the first included; n > 1 keeps the events' own cost out of most
- * launches).  mrlbm_last_step_ms: mean time per step of the last mrlbm_step call between
- * HIP events on the handle's stream (0 when timing is off). */
+/* Timing mode: on = 0 off; on = n >= 1 takes the start and end times of every n-th bulk
+ * launch of each mrlbm_step call (the first included) into a HIP event pair given to the
+ * launch itself (hipExtLaunchKernel).  mrlbm_last_step_ms: mean time per step of the last
+ * mrlbm_step call between HIP events on the handle's stream (0 when timing is off). */
 int mrlbm_set_timing(mrlbm_handle *h, int32_t on);
 int mrlbm_last_step_ms(const mrlbm_handle *h, double *ms);
 /* Mean duration (ms) of one bulk-kernel launch over the last timed mrlbm_step call (a
- * two-step launch covers two iterations), from the HIP event pairs of the timed launches on
- * the bulk stream (at most 4096); *launches = number of launches timed.
+ * two-step launch covers two iterations), from the start/end event pairs of the timed
+ * launches (at most 4096); *launches = number of launches timed.
  * New: no counterpart in the reference (its loop prints MLUPS only, 2d_mrlbm.f90:540-553). */
 int mrlbm_last_bulk_ms(const mrlbm_handle *h, double *ms_per_launch, int32_t *launches);
 

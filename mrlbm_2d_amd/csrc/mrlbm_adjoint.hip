@@ -33,7 +33,9 @@ struct AdjointWork {
     bool f32 = false;
     int K = 0, nseg_cap = 0;
     void *ckpt = nullptr;      // nseg_cap states of 6 planes
-    void *seg = nullptr;       // K + 1 states
+    void *seg[2] = {nullptr, nullptr}; // K + 1 states each; seg[1] only when memory allows
+    void *U2 = nullptr, *ub2 = nullptr; // wall scratch of the recompute stream
+    hipEvent_t evR = nullptr, evV = nullptr; // recompute done / reverse done
     void *lam[2] = {nullptr, nullptr};
     double *G = nullptr;       // dJ/dchi, one plane
     double *guin = nullptr;    // dJ/d(uin_j), ny
@@ -409,6 +411,8 @@ void adjoint_free(mrlbm_handle *h)
     if (!h || !h->adj) return;
     for (void *p : h->adj->allocs)
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : {h->adj->evR, h->adj->evV})
+        if (e) (void)hipEventDestroy(e);
     delete h->adj;
     h->adj = nullptr;
 }
@@ -439,8 +443,23 @@ template <class S, class T> static int adjoint_setup(mrlbm_handle *h, int K, int
     const size_t st_words = (size_t)6 * plane;
     w->ckpt = walloc<S>(w, st_words * nseg, e);
     HIPCHK(e);
-    w->seg = walloc<S>(w, st_words * (K + 1), e);
+    w->seg[0] = walloc<S>(w, st_words * (K + 1), e);
     HIPCHK(e);
+    { // a second segment buffer lets segment s-1 be recomputed while segment s is reversed
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        const size_t seg_b = st_words * (K + 1) * sizeof(S);
+        if (free_b > 2 * seg_b + ((size_t)1 << 30)) {
+            w->seg[1] = walloc<S>(w, st_words * (K + 1), e);
+            HIPCHK(e);
+            w->U2 = walloc<T>(w, (size_t)5 * std::max(bt.nU, 1), e);
+            HIPCHK(e);
+            w->ub2 = walloc<T>(w, (size_t)2 * std::max(bt.nbct, 1), e);
+            HIPCHK(e);
+            HIPCHK(hipEventCreateWithFlags(&w->evR, hipEventDisableTiming | hipEventReleaseToDevice));
+            HIPCHK(hipEventCreateWithFlags(&w->evV, hipEventDisableTiming | hipEventReleaseToDevice));
+        }
+    }
     for (int b = 0; b < 2; b++) {
         w->lam[b] = walloc<T>(w, st_words, e);
         HIPCHK(e);
@@ -560,18 +579,50 @@ static int adjoint_typed(mrlbm_handle *h, const mrlbm_adjoint_opts *o, double *d
     int lc = 0;
     const T wJ = (T)(1.0 / ((double)window * norm));
     const int simple_blocks = (int)(((long long)h->nx * h->ny + 255) / 256);
-    for (int s = nseg - 1; s >= 0; s--) {
+    // Segment s's states are recomputed from its checkpoint before it is reversed.  With two
+    // segment buffers the recompute of segment s-1 runs on stream B (its own wall scratch)
+    // while stream A reverses segment s; B waits for the reverse that last used its buffer.
+    const bool pipe = w->seg[1] != nullptr;
+    auto recompute = [&](int s, hipStream_t q) -> int {
         const int len = std::min(K, nsteps - s * K);
-        S *seg = (S *)w->seg;
-        HIPCHK(hipMemcpyAsync(seg, (S *)w->ckpt + st_words * s, st_words * sizeof(S), hipMemcpyDeviceToDevice, h->st));
-        for (int t = 0; t < len; t++) { // recompute the segment's states
+        S *seg = (S *)w->seg[pipe ? (s & 1) : 0];
+        HIPCHK(hipMemcpyAsync(seg, (S *)w->ckpt + st_words * s, st_words * sizeof(S), hipMemcpyDeviceToDevice, q));
+        for (int t = 0; t < len; t++) {
             auto a = base_args<S, T>(h);
             a.src = seg + st_words * t;
             a.dst = seg + st_words * (t + 1);
             a.pops = nullptr;
-            HIPCHK(launch_bulk<S, T>(a, simple_blocks, h->st));
-            HIPCHK(launch_edges<S, T>(a, h->st));
-            HIPCHK(launch_boundary<S, T>(a, h->st));
+            if (pipe) {
+                a.b.U = w->U2;
+                a.b.ub = w->ub2;
+            }
+            HIPCHK(launch_bulk<S, T>(a, simple_blocks, q));
+            HIPCHK(launch_edges<S, T>(a, q));
+            HIPCHK(launch_boundary<S, T>(a, q));
+        }
+        return 0;
+    };
+    if (pipe) {
+        HIPCHK(hipEventRecord(h->evA, h->st)); // the memsets above
+        HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
+        const int rc = recompute(nseg - 1, h->st2);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(w->evR, h->st2));
+    }
+    for (int s = nseg - 1; s >= 0; s--) {
+        const int len = std::min(K, nsteps - s * K);
+        S *seg = (S *)w->seg[pipe ? (s & 1) : 0];
+        if (pipe) {
+            HIPCHK(hipStreamWaitEvent(h->st, w->evR, 0)); // segment s recomputed
+            if (s > 0) { // segment s-1 into the buffer the reverse of segment s+1 used
+                if (s + 1 < nseg) HIPCHK(hipStreamWaitEvent(h->st2, w->evV, 0));
+                const int rc = recompute(s - 1, h->st2);
+                if (rc) return rc;
+                HIPCHK(hipEventRecord(w->evR, h->st2));
+            }
+        } else {
+            const int rc = recompute(s, h->st);
+            if (rc) return rc;
         }
         for (int t = len - 1; t >= 0; t--) {
             const int step = s * K + t; // state step -> step + 1
@@ -605,8 +656,10 @@ static int adjoint_typed(mrlbm_handle *h, const mrlbm_adjoint_opts *o, double *d
             if (rc) FAIL(rc, "adjoint kernel launch failed");
             lc ^= 1;
         }
+        if (pipe) HIPCHK(hipEventRecord(w->evV, h->st)); // buffer s & 1 is free again
     }
     HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipStreamSynchronize(h->st2));
     const auto t2 = clock::now();
     // ---- outputs
     std::vector<double> guin(h->ny);

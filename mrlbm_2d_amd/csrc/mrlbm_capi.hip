@@ -677,28 +677,30 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         const int nrec = (is_window(iter) ? 1 : 0) + (is_window(iter + 1) ? 1 : 0);
         return (int)rec_iters.size() + nrec <= std::max(h->rec_cap, 0) || nrec == 0;
     };
-    // timing mode: an event pair around every h->timing-th bulk launch (the first included),
-    // at most kMaxTimed launches
+    // timing mode: the start and end times of every h->timing-th bulk launch (the first
+    // included) into an event pair, at most kMaxTimed launches
     constexpr int kMaxTimed = 4096;
     int ntimed = 0, nlaunch = 0;
+    h->last_ms = h->last_bulk_ms = 0.0;
+    h->last_bulk_n = 0;
     if (h->timing) {
         const size_t want = 2 * (size_t)std::min(nsteps / h->timing + 1, kMaxTimed);
         while (h->tev.size() < want) {
-            hipEvent_t e; // timing only: read after e1 has synchronized with the work
-            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+            hipEvent_t e; // read after e1 has synchronized with the work
+            HIPCHK(hipEventCreate(&e));
             h->tev.push_back(e);
         }
         HIPCHK(hipEventRecord(h->e0, h->st));
     }
-    auto mark = [&](int which) -> hipError_t {
-        if (!h->timing) return hipSuccess;
-        const bool timed = nlaunch % h->timing == 0 && ntimed < kMaxTimed;
-        hipError_t e = timed ? hipEventRecord(h->tev[2 * ntimed + which], h->st) : hipSuccess;
-        if (which == 1) {
-            ntimed += timed ? 1 : 0;
-            nlaunch++;
+    // the event pair of the next bulk launch (null when it is not timed)
+    auto timed_pair = [&](hipEvent_t &t0, hipEvent_t &t1) {
+        t0 = t1 = nullptr;
+        if (h->timing && nlaunch % h->timing == 0 && ntimed < kMaxTimed) {
+            t0 = h->tev[2 * ntimed];
+            t1 = h->tev[2 * ntimed + 1];
+            ntimed++;
         }
-        return e;
+        nlaunch++;
     };
     for (int s = 0; s < nsteps; s++) {
         const int iter = iter_first + s;
@@ -712,9 +714,9 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             bulk.ncg = h->ncg2;
             bulk.ncb = h->ncb2;
             HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
-            HIPCHK(mark(0));
-            HIPCHK(launch_bulk2<S, T>(bulk, h->ncb2 * h->nstrips2, h->tb_H, h->st));
-            HIPCHK(mark(1));
+            hipEvent_t t0, t1;
+            timed_pair(t0, t1);
+            HIPCHK(launch_bulk2<S, T>(bulk, h->ncb2 * h->nstrips2, h->tb_H, h->st, t0, t1));
             HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
             a1.nrect = h->nrect1;
             std::memcpy(a1.rect, h->rect1, sizeof h->rect1);
@@ -739,9 +741,9 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         const int nb = (pops_step || a.simple || a.chi) ? simple_blocks : march_blocks;
         // bulk_n after wall_{n-1}; wall_n after bulk_{n-1}: both read state n-1 only
         HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
-        HIPCHK(mark(0));
-        HIPCHK(launch_bulk<S, T>(a, nb, h->st));
-        HIPCHK(mark(1));
+        hipEvent_t t0, t1;
+        timed_pair(t0, t1);
+        HIPCHK(launch_bulk<S, T>(a, nb, h->st, t0, t1));
         HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
         HIPCHK(side_work(a));
         HIPCHK(hipEventRecord(h->evB, h->st2));

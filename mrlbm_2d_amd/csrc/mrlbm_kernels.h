@@ -62,7 +62,8 @@ template <class S, class T> struct StepArgs {
 
 // Host launchers (mrlbm.hip).
 template <class S, class T>
-hipError_t launch_bulk(const StepArgs<S, T> &a, int nblocks, hipStream_t st);
+hipError_t launch_bulk(const StepArgs<S, T> &a, int nblocks, hipStream_t st, hipEvent_t t0 = nullptr,
+                       hipEvent_t t1 = nullptr);
 template <class S, class T>
 hipError_t launch_boundary(const StepArgs<S, T> &a, hipStream_t st);
 template <class S, class T>
@@ -80,7 +81,8 @@ hipError_t launch_stencil_moments(const StepArgs<S, T> &a, hipStream_t st);
 // Two steps in one pass over the plain interior (nodes two or more away from the edges and
 // from the dilated cylinder box), src = state t, dst = state t+2; strips of H rows.
 template <class S, class T>
-hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st);
+hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0 = nullptr,
+                        hipEvent_t t1 = nullptr);
 // Side work of one step: ring + band rectangles (+ the wall's stencil nodes when stencil),
 // and the boundary workgroup.
 template <class S, class T>

@@ -16,6 +16,8 @@
 
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 namespace mrlbm {
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b -> XCD b % 8), each XCD with
@@ -898,12 +900,22 @@ __global__ void __launch_bounds__(256) ghost_pops_kernel(StepArgs<S, T> a, T *gh
 }
 
 // ---------------------------------------------------------------------------------
-template <class S, class T> hipError_t launch_bulk(const StepArgs<S, T> &a, int nblocks, hipStream_t st)
+// Launch on stream st; with t0/t1 (timing) the two events take the kernel's own start and
+// end times (no packets of their own between the kernels).
+template <class K, class... A>
+static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t t0, hipEvent_t t1, A... args)
+{
+    if (t0) hipExtLaunchKernelGGL(kernel, g, b, 0, st, t0, t1, 0, args...);
+    else hipLaunchKernelGGL(kernel, g, b, 0, st, args...);
+}
+
+template <class S, class T>
+hipError_t launch_bulk(const StepArgs<S, T> &a, int nblocks, hipStream_t st, hipEvent_t t0, hipEvent_t t1)
 {
     const bool rec = a.sample || (a.rec_slot >= 0 && a.nprobe > 0);
     const dim3 g(nblocks), b(256);
-#define MRLBM_BULK(R, H) hipLaunchKernelGGL((bulk_kernel<S, T, R, H>), g, b, 0, st, a)
-    if (a.simple || a.pops || a.chi) hipLaunchKernelGGL((bulk_simple_kernel<S, T>), g, b, 0, st, a);
+#define MRLBM_BULK(R, H) launch_timed(bulk_kernel<S, T, R, H>, g, b, st, t0, t1, a)
+    if (a.simple || a.pops || a.chi) launch_timed(bulk_simple_kernel<S, T>, g, b, st, t0, t1, a);
     else if (rec && a.variant >= 300) MRLBM_BULK(true, 304);
     else if (rec) MRLBM_BULK(true, 0);
     else if (a.variant == 4) MRLBM_BULK(false, 4);
@@ -984,16 +996,17 @@ hipError_t launch_bulk2_batch(const StepArgs<S, T> *args, const int2 *tiles, int
     }
     return hipGetLastError();
 }
-template <class S, class T> hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st)
+template <class S, class T>
+hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0, hipEvent_t t1)
 {
     const dim3 g(nblocks), b(256);
     switch (H) {
-    case 2: hipLaunchKernelGGL((bulk2_kernel<S, T, 2>), g, b, 0, st, a); break;
-    case 4: hipLaunchKernelGGL((bulk2_kernel<S, T, 4>), g, b, 0, st, a); break;
-    case 6: hipLaunchKernelGGL((bulk2_kernel<S, T, 6>), g, b, 0, st, a); break;
-    case 8: hipLaunchKernelGGL((bulk2_kernel<S, T, 8>), g, b, 0, st, a); break;
-    case 10: hipLaunchKernelGGL((bulk2_kernel<S, T, 10>), g, b, 0, st, a); break;
-    case 12: hipLaunchKernelGGL((bulk2_kernel<S, T, 12>), g, b, 0, st, a); break;
+    case 2: launch_timed(bulk2_kernel<S, T, 2>, g, b, st, t0, t1, a); break;
+    case 4: launch_timed(bulk2_kernel<S, T, 4>, g, b, st, t0, t1, a); break;
+    case 6: launch_timed(bulk2_kernel<S, T, 6>, g, b, st, t0, t1, a); break;
+    case 8: launch_timed(bulk2_kernel<S, T, 8>, g, b, st, t0, t1, a); break;
+    case 10: launch_timed(bulk2_kernel<S, T, 10>, g, b, st, t0, t1, a); break;
+    case 12: launch_timed(bulk2_kernel<S, T, 12>, g, b, st, t0, t1, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1058,11 +1071,11 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
 }
 
 #define MRLBM_INSTANTIATE(S, T)                                                                  \
-    template hipError_t launch_bulk<S, T>(const StepArgs<S, T> &, int, hipStream_t);             \
+    template hipError_t launch_bulk<S, T>(const StepArgs<S, T> &, int, hipStream_t, hipEvent_t, hipEvent_t); \
     template hipError_t launch_boundary<S, T>(const StepArgs<S, T> &, hipStream_t);              \
     template hipError_t launch_edges<S, T>(const StepArgs<S, T> &, hipStream_t);                 \
     template hipError_t launch_stencil_moments<S, T>(const StepArgs<S, T> &, hipStream_t);       \
-    template hipError_t launch_bulk2<S, T>(const StepArgs<S, T> &, int, int, hipStream_t);       \
+    template hipError_t launch_bulk2<S, T>(const StepArgs<S, T> &, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
     template hipError_t launch_band<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
     template hipError_t launch_side<S, T>(const StepArgs<S, T> &, hipStream_t, bool);            \
     template hipError_t launch_wall<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
