@@ -31,7 +31,7 @@ struct mrlbm_batch {
     int tb = 0, ntiles2 = 0;      // two steps per bulk pass: strip height (0 off), tiles
     bool tb_possible = true;      // every run has the two-step scratch state
     int max_side[3] = {0, 0, 0};  // side-kernel threads per run: single step, sub-step 1, 2
-    bool any_wall = false;
+    int max_nbct = 0;             // largest wall of any run (0: no run has a cylinder)
     int64_t nodes = 0;
     hipStream_t st = nullptr, st2 = nullptr;
     hipEvent_t evA = nullptr, evB = nullptr, e0 = nullptr, e1 = nullptr;
@@ -93,7 +93,7 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
     std::vector<StepArgs<S, T>> args[2][2], b2[2], s1[2][2], s2[2][2];
     std::vector<int2> tiles, tiles2;
     for (int k = 0; k < 3; k++) b->max_side[k] = 0;
-    b->any_wall = false;
+    b->max_nbct = 0;
     for (int r = 0; r < R; r++) {
         mrlbm_handle *h = b->runs[r];
         const int ring = 2 * h->nx + 2 * (h->ny - 2), nU = h->has_b ? h->bt.nU : 0;
@@ -103,7 +103,7 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
             return t;
         };
         b->max_side[0] = std::max(b->max_side[0], ring + nU);
-        b->any_wall = b->any_wall || h->has_b;
+        if (h->has_b) b->max_nbct = std::max(b->max_nbct, h->bt.nbct);
         for (int par = 0; par < 2; par++)
             for (int rec = 0; rec < 2; rec++) {
                 auto a = base_args<S, T>(h);
@@ -172,7 +172,7 @@ template <class S, class T> static int launch_step(mrlbm_batch *b, int par, int 
     BHIP(hipStreamWaitEvent(b->st, b->evB, 0));
     BHIP(launch_bulk_batch<S, T>(args, b->d_tiles, b->ntiles, b->runs[0]->variant, b->grid, b->st));
     BHIP(hipStreamWaitEvent(b->st2, b->evA, 0));
-    BHIP(launch_side_batch<S, T>(args, R, b->max_side[0], b->any_wall, b->st2));
+    BHIP(launch_side_batch<S, T>(args, R, b->max_side[0], b->max_nbct, b->st2));
     BHIP(hipEventRecord(b->evB, b->st2));
     BHIP(hipEventRecord(b->evA, b->st));
     return 0;
@@ -186,8 +186,8 @@ template <class S, class T> static int launch_pair(mrlbm_batch *b, int par, int 
     BHIP(hipStreamWaitEvent(b->st, b->evB, 0));
     BHIP(launch_bulk2_batch<S, T>((const StepArgs<S, T> *)b->d_b2[par], b->d_tiles2, b->ntiles2, b->tb, b->st));
     BHIP(hipStreamWaitEvent(b->st2, b->evA, 0));
-    BHIP(launch_side_batch<S, T>((const StepArgs<S, T> *)b->d_s1[par][rec], R, b->max_side[1], b->any_wall, b->st2));
-    BHIP(launch_side_batch<S, T>((const StepArgs<S, T> *)b->d_s2[par][rec], R, b->max_side[2], b->any_wall, b->st2));
+    BHIP(launch_side_batch<S, T>((const StepArgs<S, T> *)b->d_s1[par][rec], R, b->max_side[1], b->max_nbct, b->st2));
+    BHIP(launch_side_batch<S, T>((const StepArgs<S, T> *)b->d_s2[par][rec], R, b->max_side[2], b->max_nbct, b->st2));
     BHIP(hipEventRecord(b->evB, b->st2));
     BHIP(hipEventRecord(b->evA, b->st));
     return 0;

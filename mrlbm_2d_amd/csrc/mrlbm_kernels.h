@@ -97,7 +97,7 @@ template <class S, class T>
 hipError_t launch_bulk_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int variant, int grid,
                              hipStream_t st);
 template <class S, class T>
-hipError_t launch_side_batch(const StepArgs<S, T> *args, int nruns, int max_side, int any_wall, hipStream_t st);
+hipError_t launch_side_batch(const StepArgs<S, T> *args, int nruns, int max_side, int max_nbct, hipStream_t st);
 template <class S, class T>
 hipError_t launch_bulk2_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int H, hipStream_t st);
 // Resident blocks of the batched bulk kernel of a variant on the whole device (0 on error).

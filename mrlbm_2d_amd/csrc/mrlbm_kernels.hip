@@ -748,9 +748,8 @@ template <class S, class T> __device__ void wall_records(const StepArgs<S, T> &a
     }
 }
 
-// All boundary nodes of one step in one workgroup: velocities (by dependency level), nodes,
-// records.
-template <class S, class T> __device__ void boundary_body(const StepArgs<S, T> &a)
+// All boundary nodes of one step in one workgroup: velocities (by dependency level), nodes.
+template <class S, class T> __device__ void wall_nodes_block(const StepArgs<S, T> &a)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
     if (a.b.nlevels <= 1) {
@@ -759,6 +758,12 @@ template <class S, class T> __device__ void boundary_body(const StepArgs<S, T> &
         wall_velocities(a);
         for (int k = tid; k < a.b.nbct; k += nt) wall_node<S, T, false>(a, k);
     }
+}
+
+// ... and the records of the step.
+template <class S, class T> __device__ void boundary_body(const StepArgs<S, T> &a)
+{
+    wall_nodes_block(a);
     __syncthreads();
     wall_records(a);
 }
@@ -812,6 +817,8 @@ __global__ void __launch_bounds__(256) bulk_batch_kernel(const StepArgs<S, T> *a
     }
 }
 
+// The wall of every run, one workgroup per run (measured faster here than single-wave
+// workgroups over the largest run's wall, most of which would exit at once).
 template <class S, class T> __global__ void __launch_bounds__(256) boundary_batch_kernel(const StepArgs<S, T> *args)
 {
     const StepArgs<S, T> &a = args[blockIdx.x];
@@ -976,10 +983,10 @@ template <class S, class T> int bulk_batch_resident_blocks(int variant)
     return per_cu * ncu;
 }
 template <class S, class T>
-hipError_t launch_side_batch(const StepArgs<S, T> *args, int nruns, int max_side, int any_wall, hipStream_t st)
+hipError_t launch_side_batch(const StepArgs<S, T> *args, int nruns, int max_side, int max_nbct, hipStream_t st)
 {
     hipLaunchKernelGGL((side_batch_kernel<S, T>), dim3((max_side + 255) / 256, nruns), dim3(256), 0, st, args);
-    if (any_wall) hipLaunchKernelGGL((boundary_batch_kernel<S, T>), dim3(nruns), dim3(256), 0, st, args);
+    if (max_nbct > 0) hipLaunchKernelGGL((boundary_batch_kernel<S, T>), dim3(nruns), dim3(256), 0, st, args);
     return hipGetLastError();
 }
 template <class S, class T>
