@@ -191,7 +191,7 @@ static void free_all(mrlbm_handle *h)
     if (h->e1) (void)hipEventDestroy(h->e1);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     h->tev.clear();
-    for (hipEvent_t e : {h->evA, h->evB, h->evS, h->evW})
+    for (hipEvent_t e : {h->evA, h->evB, h->evS, h->evW, h->evK})
         if (e) (void)hipEventDestroy(e);
     for (hipStream_t q : {h->st, h->st2, h->st3})
         if (q) (void)hipStreamDestroy(q);
@@ -347,11 +347,12 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
     // (no system-scope cache writeback each time one is recorded)
     const unsigned order_flags = hipEventDisableTiming | hipEventReleaseToDevice;
     if (hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess ||
-        hipEventCreateWithFlags(&h->evA, order_flags) != hipSuccess ||
+        hipEventCreateWithFlags(&h->evA, order_flags) != hipSuccess || hipEventCreate(&h->evK) != hipSuccess ||
         hipEventCreateWithFlags(&h->evB, order_flags) != hipSuccess ||
         hipEventCreateWithFlags(&h->evS, order_flags) != hipSuccess ||
         hipEventCreateWithFlags(&h->evW, order_flags) != hipSuccess)
         return fail(MRLBM_E_HIP, "hipEventCreate failed");
+    h->bulk_done = h->evA;
     if (hipEventRecord(h->evA, h->st) != hipSuccess || hipEventRecord(h->evB, h->st2) != hipSuccess)
         return fail(MRLBM_E_HIP, "hipEventRecord failed");
     const int rc = h->f32 ? create_typed<float, float>(h) : create_typed<double, double>(h);
@@ -702,6 +703,23 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         }
         nlaunch++;
     };
+    // A bulk launch ends in its own stop event (a timing event, else evK): the side work of
+    // the next step waits on it, with no marker packet between two bulk launches.  The wait
+    // on the previous launch's event is enqueued before the next launch reuses evK.
+    auto launch_bulk_pair = [&](const StepArgs<S, T> &bulk) -> hipError_t {
+        hipEvent_t t0, t1;
+        timed_pair(t0, t1);
+        if (!t1) t1 = h->evK;
+        h->bulk_done = t1;
+        return launch_bulk2<S, T>(bulk, h->ncb2 * h->nstrips2, h->tb_H, h->st, t0, t1);
+    };
+    auto launch_bulk_one = [&](const StepArgs<S, T> &a, int nb) -> hipError_t {
+        hipEvent_t t0, t1;
+        timed_pair(t0, t1);
+        if (!t1) t1 = h->evK;
+        h->bulk_done = t1;
+        return launch_bulk<S, T>(a, nb, h->st, t0, t1);
+    };
     for (int s = 0; s < nsteps; s++) {
         const int iter = iter_first + s;
         if (s + 1 < nsteps && pair_ok(iter)) {
@@ -713,11 +731,9 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             bulk.nstrips = h->nstrips2;
             bulk.ncg = h->ncg2;
             bulk.ncb = h->ncb2;
+            HIPCHK(hipStreamWaitEvent(h->st2, h->bulk_done, 0));
             HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
-            hipEvent_t t0, t1;
-            timed_pair(t0, t1);
-            HIPCHK(launch_bulk2<S, T>(bulk, h->ncb2 * h->nstrips2, h->tb_H, h->st, t0, t1));
-            HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
+            HIPCHK(launch_bulk_pair(bulk));
             a1.nrect = h->nrect1;
             std::memcpy(a1.rect, h->rect1, sizeof h->rect1);
             HIPCHK(side_work(a1));
@@ -728,7 +744,6 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             HIPCHK(side_work(a2));
             after_iter(iter + 1);
             HIPCHK(hipEventRecord(h->evB, h->st2));
-            HIPCHK(hipEventRecord(h->evA, h->st));
             h->cur ^= 1;
             record_iter(iter);
             record_iter(iter + 1);
@@ -740,14 +755,11 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         const bool pops_step = h->from_pops;
         const int nb = (pops_step || a.simple || a.chi) ? simple_blocks : march_blocks;
         // bulk_n after wall_{n-1}; wall_n after bulk_{n-1}: both read state n-1 only
+        HIPCHK(hipStreamWaitEvent(h->st2, h->bulk_done, 0));
         HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
-        hipEvent_t t0, t1;
-        timed_pair(t0, t1);
-        HIPCHK(launch_bulk<S, T>(a, nb, h->st, t0, t1));
-        HIPCHK(hipStreamWaitEvent(h->st2, h->evA, 0));
+        HIPCHK(launch_bulk_one(a, nb));
         HIPCHK(side_work(a));
         HIPCHK(hipEventRecord(h->evB, h->st2));
-        HIPCHK(hipEventRecord(h->evA, h->st));
         after_iter(iter);
         h->cur ^= 1;
         h->from_pops = false;

@@ -907,12 +907,13 @@ __global__ void __launch_bounds__(256) ghost_pops_kernel(StepArgs<S, T> a, T *gh
 }
 
 // ---------------------------------------------------------------------------------
-// Launch on stream st; with t0/t1 (timing) the two events take the kernel's own start and
-// end times (no packets of their own between the kernels).
+// Launch on stream st; t0/t1 (either may be null) take the kernel's own start and end times
+// (no packets of their own between kernels): for timing, and t1 as the launch's completion
+// event that another stream can wait on.
 template <class K, class... A>
 static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t t0, hipEvent_t t1, A... args)
 {
-    if (t0) hipExtLaunchKernelGGL(kernel, g, b, 0, st, t0, t1, 0, args...);
+    if (t0 || t1) hipExtLaunchKernelGGL(kernel, g, b, 0, st, t0, t1, 0, args...);
     else hipLaunchKernelGGL(kernel, g, b, 0, st, args...);
 }
 
