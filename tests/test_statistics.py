@@ -79,3 +79,15 @@ def test_drop_in_program_reads_and_writes_the_reference_files(name, tmp_path):
     cd_t = np.loadtxt(tmp_path / "plots" / "cd_time.dat")
     assert cd_t[0, 0] == 0.0 and np.all(np.diff(cd_t[:, 0]) > 0)
     assert os.path.exists(tmp_path / "plots" / "cl_time.dat")
+
+
+def test_spectral_st_of_a_known_period_with_ripple():
+    """compare.spectral_st recovers St = D / (period * u) of a pure tone within 1e-3 and is
+    not moved by a step-to-step ripple that gives the first-two-maxima rule period 2."""
+    from mrlbm_2d_amd.compare import spectral_st
+    n, period, D, u = 100_000, 3642.7, 40.0, 0.04
+    t = np.arange(n)
+    tone = 0.5 * np.sin(2 * np.pi * t / period)
+    ripple = 1e-6 * (-1.0) ** t
+    st = spectral_st(tone + ripple, D, u)
+    assert abs(st - D / (period * u)) < 1e-3 * D / (period * u)
