@@ -89,9 +89,6 @@ template <class S, class T>
 hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st, bool stencil);
 template <class S, class T>
 hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st);
-// Plain node update over a.rect[0..nrect) (boundary nodes skipped): the band sub-steps.
-template <class S, class T>
-hipError_t launch_band(const StepArgs<S, T> &a, hipStream_t st);
 // Batched runs: args[r] per run in device memory; tiles[t] = (run, bulk block of that run).
 template <class S, class T>
 hipError_t launch_bulk_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int variant, int grid,

@@ -458,24 +458,6 @@ template <class S, class T, int H> __global__ void __launch_bounds__(256) bulk2_
     bulk_march2<S, T, H>(a, blockIdx.x);
 }
 
-// Plain node update over the band rectangles of a two-step pair (boundary nodes are
-// skipped by node_update; the ring is never inside a rectangle).
-template <class S, class T> __global__ void __launch_bounds__(256) band_kernel(StepArgs<S, T> a)
-{
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int q = 0; q < a.nrect; q++) {
-        const long long n = (long long)a.rect[q][2] * a.rect[q][3];
-        if (t < n) {
-            const int i = a.rect[q][0] + (int)(t % a.rect[q][2]), j = a.rect[q][1] + (int)(t / a.rect[q][2]);
-            T f[Q];
-            gather_generic(a, i, j, f);
-            node_update(a, i, j, f, false);
-            return;
-        }
-        t -= n;
-    }
-}
-
 // The side stream's node work of one step in one launch: the outer ring (edge update),
 // the band rectangles of a two-step pair (plain update) and the curved wall's stencil
 // nodes (pre-collision moments for boundary_kernel).  All three read only the source
@@ -1019,13 +1001,6 @@ hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t
     }
     return hipGetLastError();
 }
-template <class S, class T> hipError_t launch_band(const StepArgs<S, T> &a, hipStream_t st)
-{
-    long long n = 0;
-    for (int q = 0; q < a.nrect; q++) n += (long long)a.rect[q][2] * a.rect[q][3];
-    if (n > 0) hipLaunchKernelGGL((band_kernel<S, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
-    return hipGetLastError();
-}
 template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st, bool stencil)
 {
     const int nring = 2 * a.nx + 2 * (a.ny - 2);
@@ -1084,7 +1059,6 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_edges<S, T>(const StepArgs<S, T> &, hipStream_t);                 \
     template hipError_t launch_stencil_moments<S, T>(const StepArgs<S, T> &, hipStream_t);       \
     template hipError_t launch_bulk2<S, T>(const StepArgs<S, T> &, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
-    template hipError_t launch_band<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
     template hipError_t launch_side<S, T>(const StepArgs<S, T> &, hipStream_t, bool);            \
     template hipError_t launch_wall<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
     template hipError_t launch_bulk_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, int, hipStream_t); \
