@@ -74,6 +74,51 @@ template <class S, class T> __device__ void gather_generic(const StepArgs<S, T> 
     for (int k = 0; k < Q; k++) f[k] = pull_rule(a, i, j, k);
 }
 
+// Gather at any node, with the same rules as pull_rule, but every source (upstream node,
+// periodic wrap or stale-population table entry) is located first and all loads are issued
+// before any arithmetic: a thread of the outer ring waits for one memory round trip.
+template <class S, class T> __device__ void gather_edge(const StepArgs<S, T> &a, int i, int j, T f[Q])
+{
+    if (a.pops) {
+        gather_generic(a, i, j, f);
+        return;
+    }
+    S m[Q][6];
+    T gv[Q];
+    bool stale[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        int si = i - EXk(k), sj = j - EYk(k), gi = 0;
+        const bool xin = si >= 0 && si < a.nx, yin = sj >= 0 && sj < a.ny;
+        stale[k] = false;
+        if (!(xin && yin)) {
+            if (!yin && a.yper) {
+                si = i;
+                sj = sj < 0 ? a.ny - 1 : 0;
+            } else if (!xin && a.xper) {
+                si = si < 0 ? a.nx - 1 : 0;
+                sj = j;
+            } else { // the table entry; (i, j) itself stands in for the unused moment load
+                stale[k] = true;
+                gi = si < 0 ? j * Q + k
+                   : si >= a.nx ? (a.ny + j) * Q + k
+                   : sj < 0 ? (2 * a.ny + i) * Q + k
+                            : (2 * a.ny + a.nx + i) * Q + k;
+                si = i;
+                sj = j;
+            }
+        }
+        load_moments(a, si, sj, m[k]);
+        gv[k] = a.ghost[gi];
+    }
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        T g[Q];
+        FormOf<T>::F::recon(m[k], g);
+        f[k] = stale[k] ? gv[k] : g[k];
+    }
+}
+
 // Gather at a node whose 8 neighbours are all inside the domain (the cylinder's stencil
 // and boundary nodes): all 54 loads are issued before any arithmetic, so one thread
 // waits for one memory round trip instead of nine.
@@ -114,7 +159,7 @@ __device__ void edge_update(const StepArgs<S, T> &a, int c, int j, const T fpop[
     mm[0] = F::rho_of(m[0]);
     if (out) { // rho, u copied from column nx-1 before any BC (2d_mrlbm.f90:249-253)
         T g[Q], mo[6];
-        gather_generic(a, a.nx - 2, j, g);
+        gather_edge(a, a.nx - 2, j, g);
         F::moments(g, mo);
         d_keep = mo[0];
         mm[0] = F::rho_of(mo[0]);

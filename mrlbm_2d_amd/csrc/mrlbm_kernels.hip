@@ -472,10 +472,10 @@ template <class S, class T> __device__ void side_body(const StepArgs<S, T> &a, l
     if (t < nband) {
         for (int q = 0; q < a.nrect; q++) {
             const long long n = (long long)a.rect[q][2] * a.rect[q][3];
-            if (t < n) {
+            if (t < n) { // every rectangle lies in [1, nx-2] x [1, ny-2]: all 8 neighbours inside
                 const int i = a.rect[q][0] + (int)(t % a.rect[q][2]), j = a.rect[q][1] + (int)(t / a.rect[q][2]);
                 T f[Q];
-                gather_generic(a, i, j, f);
+                gather_interior(a, i, j, f);
                 node_update(a, i, j, f, false);
                 return;
             }
@@ -499,8 +499,8 @@ template <class S, class T> __global__ void __launch_bounds__(256) side_kernel(S
     side_body(a, (long long)blockIdx.x * blockDim.x + threadIdx.x, nring, nband);
 }
 
-// Bulk, one node per thread with the generic gather: the first step after
-// mrlbm_set_populations (populations in memory) and a checking path.
+// Bulk, one node per thread: the first step after mrlbm_set_populations (populations in
+// memory), steps with a Brinkman field, and a checking path.
 template <class S, class T> __device__ void bulk_simple(const StepArgs<S, T> &a, int bb)
 {
     const long long t = (long long)bb * blockDim.x + threadIdx.x;
@@ -508,7 +508,7 @@ template <class S, class T> __device__ void bulk_simple(const StepArgs<S, T> &a,
     const int c = (int)(t % a.nx), j = (int)(t / a.nx);
     if (c == 0 || c == a.nx - 1 || j == 0 || j == a.ny - 1) return; // edge kernel
     T f[Q];
-    gather_generic(a, c, j, f);
+    gather_interior(a, c, j, f);
     node_update<S, T, true>(a, c, j, f, false);
 }
 
@@ -524,7 +524,7 @@ template <class S, class T> __device__ void edge_body(const StepArgs<S, T> &a, i
     else if (r < 2 * nx + ny - 2) { c = 0; j = 1 + r - 2 * nx; }
     else { c = nx - 1; j = 1 + r - 2 * nx - (ny - 2); }
     T f[Q];
-    gather_generic(a, c, j, f);
+    gather_edge(a, c, j, f);
     node_update(a, c, j, f, true);
 }
 
