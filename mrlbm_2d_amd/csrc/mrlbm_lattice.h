@@ -91,13 +91,13 @@ __device__ __forceinline__ void moments_f64(const double f[Q], double &r, double
 __device__ __forceinline__ void recon9_dev(float d, float ux, float uy, float mxx, float myy, float mxy,
                                            float e[Q])
 {
-    constexpr float HA = Lat<float>::HA, HB = Lat<float>::HB;
-    const float X = 4.5f * mxx, Y = 4.5f * myy, Z = 9.0f * mxy;
-    const float XA = X * HA, XB = X * HB, YA = Y * HA, YB = Y * HB;
-    const float c1 = 3.0f * ux, c2 = 3.0f * uy;
-    const float c5 = 3.0f * (ux + uy), c6 = 3.0f * (uy - ux), c8 = 3.0f * (ux - uy);
+    // e_k = w_k (d + r T_k), r = 1 + d, T_k = 3 c_k.u + 4.5 H_k:m: the 4.5 H_aa products are
+    // single constants and the diagonal velocity terms are sums of the axis ones.
+    constexpr float KA = 4.5f * Lat<float>::HA, KB = 4.5f * Lat<float>::HB;
+    const float XA = mxx * KA, XB = mxx * KB, YA = myy * KA, YB = myy * KB, Z = 9.0f * mxy;
+    const float c1 = 3.0f * ux, c2 = 3.0f * uy, c5 = c1 + c2, c6 = c2 - c1;
     const float r = 1.0f + d;
-    const float P = XB + YA, Qv = XA + YB, Bp = (XB + YB) + Z, Bm = (XB + YB) - Z;
+    const float P = XB + YA, Qv = XA + YB, B = XB + YB, Bp = B + Z, Bm = B - Z;
     e[0] = Lat<float>::w0 * __builtin_fmaf(r, XA + YA, d);
     e[1] = Lat<float>::w1 * __builtin_fmaf(r, P + c1, d);
     e[2] = Lat<float>::w1 * __builtin_fmaf(r, Qv + c2, d);
@@ -106,7 +106,7 @@ __device__ __forceinline__ void recon9_dev(float d, float ux, float uy, float mx
     e[5] = Lat<float>::w5 * __builtin_fmaf(r, Bp + c5, d);
     e[6] = Lat<float>::w5 * __builtin_fmaf(r, Bm + c6, d);
     e[7] = Lat<float>::w5 * __builtin_fmaf(r, Bp - c5, d);
-    e[8] = Lat<float>::w5 * __builtin_fmaf(r, Bm + c8, d);
+    e[8] = Lat<float>::w5 * __builtin_fmaf(r, Bm - c6, d);
 }
 
 __device__ __forceinline__ void moments_dev(const float e[Q], float &d, float &ux, float &uy, float &mxx,
@@ -116,8 +116,9 @@ __device__ __forceinline__ void moments_dev(const float e[Q], float &d, float &u
     const float a13 = e[1] + e[3], a24 = e[2] + e[4], a57 = e[5] + e[7], a68 = e[6] + e[8];
     const float diag = a57 + a68;
     d = ((e[0] + (a13 + a24)) + diag) + DEV_MASS_F;
-    const float sx = (e[1] - e[3]) + ((e[5] - e[7]) + (e[8] - e[6]));
-    const float sy = (e[2] - e[4]) + ((e[5] - e[7]) + (e[6] - e[8]));
+    const float d57 = e[5] - e[7], d68 = e[6] - e[8]; // e8 - e6 == -d68 exactly
+    const float sx = (e[1] - e[3]) + (d57 - d68);
+    const float sy = (e[2] - e[4]) + (d57 + d68);
     const float sxx = __builtin_fmaf(HA, e[0] + a24, HB * (a13 + diag));
     const float syy = __builtin_fmaf(HA, e[0] + a13, HB * (a24 + diag));
     const float sxy = a57 - a68;
