@@ -30,8 +30,11 @@ struct mrlbm_batch {
     int kernel = 0, persistent = 1, grid = 0, ntiles = 0, graph_steps = 50, tune_steps = 20;
     int tb = 0, ntiles2 = 0;      // two steps per bulk pass: strip height (0 off), tiles
     bool tb_possible = true;      // every run has the two-step scratch state
-    int max_side[3] = {0, 0, 0};  // side-kernel threads per run: single step, sub-step 1, 2
-    int max_nbct = 0;             // largest wall of any run (0: no run has a cylinder)
+    int2 *d_stiles[3] = {nullptr, nullptr, nullptr}; // side blocks (run, block): single step, sub-step 1, 2
+    int nstiles[3] = {0, 0, 0};
+    int2 *d_wtiles = nullptr;     // wall blocks (run, block of 64 boundary nodes)
+    int nwtiles = 0;
+    bool wall_levels = false;     // some wall has more than one dependency level
     int64_t nodes = 0;
     hipStream_t st = nullptr, st2 = nullptr;
     hipEvent_t evA = nullptr, evB = nullptr, e0 = nullptr, e1 = nullptr;
@@ -92,8 +95,11 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
     const int R = (int)b->runs.size();
     std::vector<StepArgs<S, T>> args[2][2], b2[2], s1[2][2], s2[2][2];
     std::vector<int2> tiles, tiles2;
-    for (int k = 0; k < 3; k++) b->max_side[k] = 0;
-    b->max_nbct = 0;
+    std::vector<int2> stiles[3], wtiles;
+    b->wall_levels = false;
+    auto add_blocks = [](std::vector<int2> &v, int r, int n) {
+        for (int k = 0; k < (n + 63) / 64; k++) v.push_back(make_int2(r, k));
+    };
     for (int r = 0; r < R; r++) {
         mrlbm_handle *h = b->runs[r];
         const int ring = 2 * h->nx + 2 * (h->ny - 2), nU = h->has_b ? h->bt.nU : 0;
@@ -102,8 +108,11 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
             for (int q = 0; q < n; q++) t += rc[q][2] * rc[q][3];
             return t;
         };
-        b->max_side[0] = std::max(b->max_side[0], ring + nU);
-        if (h->has_b) b->max_nbct = std::max(b->max_nbct, h->bt.nbct);
+        add_blocks(stiles[0], r, ring + nU);
+        if (h->has_b) {
+            add_blocks(wtiles, r, h->bt.nbct);
+            b->wall_levels = b->wall_levels || h->bt.nlevels > 1;
+        }
         for (int par = 0; par < 2; par++)
             for (int rec = 0; rec < 2; rec++) {
                 auto a = base_args<S, T>(h);
@@ -139,8 +148,8 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
         const int nb = h->ncb * h->nstrips;
         for (int t = 0; t < nb; t++) tiles.push_back(make_int2(r, t));
         if (b->tb) {
-            b->max_side[1] = std::max(b->max_side[1], ring + band(h->rect1, h->nrect1) + nU);
-            b->max_side[2] = std::max(b->max_side[2], ring + band(h->rect2, h->nrect2) + nU);
+            add_blocks(stiles[1], r, ring + band(h->rect1, h->nrect1) + nU);
+            add_blocks(stiles[2], r, ring + band(h->rect2, h->nrect2) + nU);
             const int nb2 = h->ncb2 * ((h->ny - 4 + b->tb - 1) / b->tb);
             for (int t = 0; t < nb2; t++) tiles2.push_back(make_int2(r, t));
         }
@@ -156,7 +165,11 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
     }
     if (!rc) rc = upload_tiles<T>(b, b->d_tiles, tiles);
     if (!rc) rc = upload_tiles<T>(b, b->d_tiles2, tiles2);
+    for (int k = 0; k < 3 && !rc; k++) rc = upload_tiles<T>(b, b->d_stiles[k], stiles[k]);
+    if (!rc) rc = upload_tiles<T>(b, b->d_wtiles, wtiles);
     if (rc) return rc;
+    for (int k = 0; k < 3; k++) b->nstiles[k] = (int)stiles[k].size();
+    b->nwtiles = (int)wtiles.size();
     b->ntiles = (int)tiles.size();
     b->ntiles2 = (int)tiles2.size();
     const int resident = bulk_batch_resident_blocks<S, T>(b->runs[0]->variant);
@@ -172,7 +185,8 @@ template <class S, class T> static int launch_step(mrlbm_batch *b, int par, int 
     BHIP(hipStreamWaitEvent(b->st, b->evB, 0));
     BHIP(launch_bulk_batch<S, T>(args, b->d_tiles, b->ntiles, b->runs[0]->variant, b->grid, b->st));
     BHIP(hipStreamWaitEvent(b->st2, b->evA, 0));
-    BHIP(launch_side_batch<S, T>(args, R, b->max_side[0], b->max_nbct, b->st2));
+    BHIP(launch_side_tiles<S, T>(args, b->d_stiles[0], b->nstiles[0], b->d_wtiles, b->nwtiles, R, b->wall_levels,
+                                 rec != 0, b->st2));
     BHIP(hipEventRecord(b->evB, b->st2));
     BHIP(hipEventRecord(b->evA, b->st));
     return 0;
@@ -186,8 +200,10 @@ template <class S, class T> static int launch_pair(mrlbm_batch *b, int par, int 
     BHIP(hipStreamWaitEvent(b->st, b->evB, 0));
     BHIP(launch_bulk2_batch<S, T>((const StepArgs<S, T> *)b->d_b2[par], b->d_tiles2, b->ntiles2, b->tb, b->st));
     BHIP(hipStreamWaitEvent(b->st2, b->evA, 0));
-    BHIP(launch_side_batch<S, T>((const StepArgs<S, T> *)b->d_s1[par][rec], R, b->max_side[1], b->max_nbct, b->st2));
-    BHIP(launch_side_batch<S, T>((const StepArgs<S, T> *)b->d_s2[par][rec], R, b->max_side[2], b->max_nbct, b->st2));
+    BHIP(launch_side_tiles<S, T>((const StepArgs<S, T> *)b->d_s1[par][rec], b->d_stiles[1], b->nstiles[1], b->d_wtiles,
+                                 b->nwtiles, R, b->wall_levels, rec != 0, b->st2));
+    BHIP(launch_side_tiles<S, T>((const StepArgs<S, T> *)b->d_s2[par][rec], b->d_stiles[2], b->nstiles[2], b->d_wtiles,
+                                 b->nwtiles, R, b->wall_levels, rec != 0, b->st2));
     BHIP(hipEventRecord(b->evB, b->st2));
     BHIP(hipEventRecord(b->evA, b->st));
     return 0;
@@ -388,8 +404,8 @@ static void batch_free(mrlbm_batch *b)
                 if (p) (void)hipFree(p);
         if (b->d_b2[par]) (void)hipFree(b->d_b2[par]);
     }
-    if (b->d_tiles) (void)hipFree(b->d_tiles);
-    if (b->d_tiles2) (void)hipFree(b->d_tiles2);
+    for (int2 *p : {b->d_tiles, b->d_tiles2, b->d_stiles[0], b->d_stiles[1], b->d_stiles[2], b->d_wtiles})
+        if (p) (void)hipFree(p);
     if (b->d_facc) (void)hipFree(b->d_facc);
     for (hipEvent_t e : {b->evA, b->evB, b->e0, b->e1})
         if (e) (void)hipEventDestroy(e);

@@ -93,8 +93,12 @@ hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st);
 template <class S, class T>
 hipError_t launch_bulk_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int variant, int grid,
                              hipStream_t st);
+// Side work of every run of a batch (single-wave blocks from the (run, block) tables), then
+// the walls: per-run workgroups when some wall has several dependency levels, else wall
+// blocks plus, on recording steps, one records workgroup per run.
 template <class S, class T>
-hipError_t launch_side_batch(const StepArgs<S, T> *args, int nruns, int max_side, int max_nbct, hipStream_t st);
+hipError_t launch_side_tiles(const StepArgs<S, T> *args, const int2 *stiles, int nstiles, const int2 *wtiles,
+                             int nwtiles, int nruns, bool wall_levels, bool records, hipStream_t st);
 template <class S, class T>
 hipError_t launch_bulk2_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int H, hipStream_t st);
 // Resident blocks of the batched bulk kernel of a variant on the whole device (0 on error).

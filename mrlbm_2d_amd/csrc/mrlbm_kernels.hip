@@ -799,8 +799,7 @@ __global__ void __launch_bounds__(256) bulk_batch_kernel(const StepArgs<S, T> *a
     }
 }
 
-// The wall of every run, one workgroup per run (measured faster here than single-wave
-// workgroups over the largest run's wall, most of which would exit at once).
+// The wall of every run, one workgroup per run, velocities by dependency level.
 template <class S, class T> __global__ void __launch_bounds__(256) boundary_batch_kernel(const StepArgs<S, T> *args)
 {
     const StepArgs<S, T> &a = args[blockIdx.x];
@@ -808,10 +807,30 @@ template <class S, class T> __global__ void __launch_bounds__(256) boundary_batc
     boundary_body(a);
 }
 
-template <class S, class T> __global__ void __launch_bounds__(256) side_batch_kernel(const StepArgs<S, T> *args)
+// Side work of every run from a compact table of (run, block of 64 side nodes): ring, band
+// rectangles and stencil nodes in single-wave workgroups, none of them empty.
+template <class S, class T> __global__ void __launch_bounds__(64) side_tiles_kernel(const StepArgs<S, T> *args, const int2 *tiles)
 {
-    const StepArgs<S, T> &a = args[blockIdx.y];
-    side_body(a, (long long)blockIdx.x * blockDim.x + threadIdx.x, 2 * a.nx + 2 * (a.ny - 2), band_nodes(a));
+    const int2 t = tiles[blockIdx.x];
+    const StepArgs<S, T> &a = args[t.x];
+    side_body(a, (long long)t.y * 64 + threadIdx.x, 2 * a.nx + 2 * (a.ny - 2), band_nodes(a));
+}
+
+// Wall nodes of every run from a table of (run, block of 64 boundary nodes), for walls of one
+// dependency level; the records follow in wall_records_batch_kernel.
+template <class S, class T> __global__ void __launch_bounds__(64) wall_tiles_kernel(const StepArgs<S, T> *args, const int2 *tiles)
+{
+    const int2 t = tiles[blockIdx.x];
+    const StepArgs<S, T> &a = args[t.x];
+    const int k = t.y * 64 + threadIdx.x;
+    if (k < a.b.nbct) wall_node<S, T, true>(a, k);
+}
+
+template <class S, class T> __global__ void __launch_bounds__(256) wall_records_batch_kernel(const StepArgs<S, T> *args)
+{
+    const StepArgs<S, T> &a = args[blockIdx.x];
+    if (!a.has_boundary) return;
+    wall_records(a);
 }
 
 template <class S, class T, int H>
@@ -966,10 +985,16 @@ template <class S, class T> int bulk_batch_resident_blocks(int variant)
     return per_cu * ncu;
 }
 template <class S, class T>
-hipError_t launch_side_batch(const StepArgs<S, T> *args, int nruns, int max_side, int max_nbct, hipStream_t st)
+hipError_t launch_side_tiles(const StepArgs<S, T> *args, const int2 *stiles, int nstiles, const int2 *wtiles,
+                             int nwtiles, int nruns, bool wall_levels, bool records, hipStream_t st)
 {
-    hipLaunchKernelGGL((side_batch_kernel<S, T>), dim3((max_side + 255) / 256, nruns), dim3(256), 0, st, args);
-    if (max_nbct > 0) hipLaunchKernelGGL((boundary_batch_kernel<S, T>), dim3(nruns), dim3(256), 0, st, args);
+    if (nstiles > 0) hipLaunchKernelGGL((side_tiles_kernel<S, T>), dim3(nstiles), dim3(64), 0, st, args, stiles);
+    if (wall_levels) { // some wall interpolates in place by level: one workgroup per run
+        hipLaunchKernelGGL((boundary_batch_kernel<S, T>), dim3(nruns), dim3(256), 0, st, args);
+    } else if (nwtiles > 0) {
+        hipLaunchKernelGGL((wall_tiles_kernel<S, T>), dim3(nwtiles), dim3(64), 0, st, args, wtiles);
+        if (records) hipLaunchKernelGGL((wall_records_batch_kernel<S, T>), dim3(nruns), dim3(256), 0, st, args);
+    }
     return hipGetLastError();
 }
 template <class S, class T>
@@ -1062,7 +1087,8 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_side<S, T>(const StepArgs<S, T> &, hipStream_t, bool);            \
     template hipError_t launch_wall<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
     template hipError_t launch_bulk_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, int, hipStream_t); \
-    template hipError_t launch_side_batch<S, T>(const StepArgs<S, T> *, int, int, int, hipStream_t); \
+    template hipError_t launch_side_tiles<S, T>(const StepArgs<S, T> *, const int2 *, int, const int2 *, int, int, bool, \
+                                                bool, hipStream_t);                                  \
     template hipError_t launch_bulk2_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, hipStream_t); \
     template int bulk_batch_resident_blocks<S, T>(int);                                          \
     template hipError_t launch_inlet_fold<S, T>(const StepArgs<S, T> &, hipStream_t);            \
