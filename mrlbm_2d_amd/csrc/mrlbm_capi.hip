@@ -618,8 +618,9 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         return a;
     };
     // Side work of one iteration on stream B: the ring and band kernel there, and in
-    // parallel on stream C the curved wall (stencil nodes, then the boundary workgroup) or,
-    // without a cylinder, the inlet-density fold; stream B then waits for stream C.
+    // parallel on stream C the curved wall (stencil nodes, then the wall nodes and, on
+    // recording steps, the records) or, without a cylinder, the inlet-density fold; stream B
+    // then waits for stream C.
     // Small grids are launch-bound: there the wall chain stays on stream B after the ring and
     // band kernel (which then also does the stencil nodes), saving the fork and join.
     const bool split_wall = (long long)h->nx * h->ny >= (1LL << 22);
