@@ -175,7 +175,11 @@ typedef struct mrlbm_adjoint_opts {
     int32_t window;               /* J = mean Cd over the last `window` steps */
     int32_t checkpoint_interval;  /* K: moment fields are checkpointed every K steps and the
                                      K steps in between are recomputed in the reverse sweep */
-    int32_t reserved[4];
+    int32_t serial_recompute;     /* 0 (default): when device memory holds a second segment
+                                     buffer, segment s-1 is recomputed on a second stream while
+                                     segment s is reversed; 1: one buffer, one stream.  Same
+                                     result either way */
+    int32_t reserved[3];
 } mrlbm_adjoint_opts;
 
 typedef struct mrlbm_adjoint_result {

@@ -73,7 +73,7 @@ class Info(C.Structure):
 class AdjointOpts(C.Structure):
     _fields_ = [
         ("iter_first", C.c_int32), ("nsteps", C.c_int32), ("window", C.c_int32),
-        ("checkpoint_interval", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("checkpoint_interval", C.c_int32), ("serial_recompute", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -274,11 +274,13 @@ class Lattice:
         arr = _fort(chi, (self.nx, self.ny)) if chi is not None else None
         self._check(self._L.mrlbm_set_brinkman(self._h, _dp(arr)))
 
-    def adjoint_drag(self, iter_first: int, nsteps: int, window: int, checkpoint_interval: int):
+    def adjoint_drag(self, iter_first: int, nsteps: int, window: int, checkpoint_interval: int,
+                     serial_recompute: bool = False):
         """Forward run with checkpoints, then the adjoint sweep.  Returns (result, dJ/dchi)
         with dJ/dchi as an (nx, ny) array."""
         o = default_adjoint_opts()
         o.iter_first, o.nsteps, o.window, o.checkpoint_interval = iter_first, nsteps, window, checkpoint_interval
+        o.serial_recompute = 1 if serial_recompute else 0
         g = np.zeros(self.N)
         r = AdjointResult()
         self._check(self._L.mrlbm_adjoint_drag(self._h, C.byref(o), _dp(g), C.byref(r)))

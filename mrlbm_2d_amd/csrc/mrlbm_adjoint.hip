@@ -582,7 +582,7 @@ static int adjoint_typed(mrlbm_handle *h, const mrlbm_adjoint_opts *o, double *d
     // Segment s's states are recomputed from its checkpoint before it is reversed.  With two
     // segment buffers the recompute of segment s-1 runs on stream B (its own wall scratch)
     // while stream A reverses segment s; B waits for the reverse that last used its buffer.
-    const bool pipe = w->seg[1] != nullptr;
+    const bool pipe = w->seg[1] != nullptr && !o->serial_recompute;
     auto recompute = [&](int s, hipStream_t q) -> int {
         const int len = std::min(K, nsteps - s * K);
         S *seg = (S *)w->seg[pipe ? (s & 1) : 0];

@@ -38,7 +38,8 @@ module mrlbm_iface
 
     type, bind(C) :: mrlbm_adjoint_opts
         integer(c_int32_t) :: iter_first, nsteps, window, checkpoint_interval
-        integer(c_int32_t) :: reserved(4)
+        integer(c_int32_t) :: serial_recompute
+        integer(c_int32_t) :: reserved(3)
     end type
 
     type, bind(C) :: mrlbm_adjoint_result

@@ -74,6 +74,17 @@ def fp64():
     return chi, res, g, omega
 
 
+def test_serial_recompute_gives_the_same_gradient(fp64):
+    """One segment buffer on one stream, and the recompute of the next segment overlapped
+    on a second stream, run the same kernels on the same states: identical outputs."""
+    chi, res, g, _ = fp64
+    with capi.Lattice(params()) as lat:
+        lat.set_brinkman(chi)
+        res1, g1 = lat.adjoint_drag(2, NSTEPS, WINDOW, K, serial_recompute=True)
+    assert res1.J == res.J and res1.dJ_duo == res.dJ_duo
+    np.testing.assert_array_equal(g1, g)
+
+
 def test_objective_matches_forward_run(fp64):
     chi, res, _, _ = fp64
     # same forces; only the order of the window sum differs
