@@ -21,6 +21,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define Q 9
 static const int EX[Q] = {0, 1, 0, -1, 0, 1, -1, -1, 1}; /* M:130 */
@@ -723,7 +726,9 @@ void orc_step(orc_state *s, int iter)
     const orc_params *P = &s->p;
     const int nx = s->nx, ny = s->ny, N = s->N;
     const double om = s->omega;
-    /* moments (M:231-245) */
+    /* moments (M:231-245); the per-node sweeps run on OpenMP threads like the reference's
+       (each node's arithmetic is unchanged, so results do not depend on the thread count) */
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; n++) {
         double f[Q];
         for (int k = 0; k < Q; k++) f[k] = FK(s->f, n, k);
@@ -800,6 +805,7 @@ void orc_step(orc_state *s, int iter)
             s->rho_inf_mean = ((c * s->rho_inf_mean) + rinf) / (c + 1.0);
         }
         if (iter % P->iplot == 0 && (P->channel_with_cylinder || P->channel_with_square))
+#pragma omp parallel for schedule(static)
             for (int n = 0; n < N; n++)
                 if (s->issolid[n] == 1) {
                     s->ux[n] = 0.0;
@@ -807,6 +813,7 @@ void orc_step(orc_state *s, int iter)
                 }
     }
     /* collision (M:407-415) */
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; n++) {
         const double ux = s->ux[n], uy = s->uy[n];
         s->mxx[n] = (1.0 - om) * s->mxx[n] + (om * ux) * ux;
@@ -815,6 +822,7 @@ void orc_step(orc_state *s, int iter)
     }
     const double *Mxx = s->mxx, *Myy = s->myy, *Mxy = s->mxy;
     if (P->incomp_bulk) { /* trace projection (M:419-428) */
+#pragma omp parallel for schedule(static)
         for (int n = 0; n < N; n++) {
             const double ux = s->ux[n], uy = s->uy[n];
             const double tr = 0.50 * (((s->mxx[n] + s->myy[n]) - ux * ux) - uy * uy);
@@ -825,6 +833,7 @@ void orc_step(orc_state *s, int iter)
         Mxx = s->mxx_s; Myy = s->myy_s; Mxy = s->mxy_s;
     }
     /* regularized reconstruction (M:445-457) */
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; n++) {
         const double ux = s->ux[n], uy = s->uy[n], r = s->rho[n];
         for (int k = 0; k < Q; k++) {
@@ -846,7 +855,9 @@ void orc_step(orc_state *s, int iter)
         s->F_lift_net = sl;
     }
     /* streaming: copy, push, shifted periodic wraps (M:508-535) */
-    memcpy(s->ftmp, s->fout, sizeof(double) * 9 * (size_t)N);
+#pragma omp parallel for schedule(static)
+    for (long long q = 0; q < 9LL * N; q++) s->ftmp[q] = s->fout[q];
+#pragma omp parallel for schedule(static) /* every target is written by one source */
     for (int i = 0; i < nx; i++)
         for (int j = 0; j < ny; j++)
             for (int k = 0; k < Q; k++) {
@@ -885,6 +896,15 @@ void orc_get_surface_p(const orc_state *s, double *ps, double *pfit)
 }
 
 double orc_rho_inf_mean(const orc_state *s) { return s->rho_inf_mean; }
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 int orc_statistics(int nstep, const double *fd, const double *fl, double uo, double rho_infty,
                    double r_cy, int *cycle_start, int *period, int *n_cycle,

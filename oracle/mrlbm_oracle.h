@@ -66,6 +66,8 @@ void orc_run(orc_state *s, int iter_first, int nsteps);
 void orc_get_forces(const orc_state *s, double *fd, double *fl);
 void orc_get_surface_p(const orc_state *s, double *ps, double *pfit);
 double orc_rho_inf_mean(const orc_state *s);
+/* OpenMP threads the per-node sweeps use (1 without OpenMP). */
+int orc_num_threads(void);
 
 /* 2d_statistics.f90 post-processor restated: peaks of the lift series, St, mean Cd and
  * rms Cl (2d_statistics.f90:90-131, 183-214).  Returns 0, or -1 when fewer than two peaks. */

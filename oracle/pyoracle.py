@@ -101,8 +101,15 @@ def lib():
         L.orc_quadratic_interpolation.restype = C.c_double
         L.orc_quadratic_interpolation.argtypes = [C.c_double] * 11
         L.orc_solve_system.argtypes = [dp, dp, dp]
+        L.orc_num_threads.restype = C.c_int
+        L.orc_num_threads.argtypes = []
         _lib = L
     return _lib
+
+
+def num_threads() -> int:
+    """OpenMP threads of the oracle's per-node sweeps (OMP_NUM_THREADS, else all cores)."""
+    return int(lib().orc_num_threads())
 
 
 def _dp(a):
