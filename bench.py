@@ -33,10 +33,10 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel at 8192 x 2048 (the two-step bulk pass, two
-# steps per launch: FP32 strips of 10 rows, FP64 of 8), from rocprofv3 FETCH_SIZE and
+# steps per launch: FP32 strips of 10 rows, FP64 of 12), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1);
 # the summary and the commands are in profiles/bulk2_kernel_8192x2048_hbm_bytes.md
-TRAFFIC_BYTES = {"f32": 947_798_552, "f64": 1_797_846_091}
+TRAFFIC_BYTES = {"f32": 949_914_039, "f64": 2_288_934_108}
 NX, NY = 8192, 2048
 # HIP event pairs around every 4th bulk launch: the per-launch mean without the events' own
 # cost (about 2% of the step when recorded around every launch)
@@ -270,7 +270,7 @@ def main():
     default_shape = args.strip_rows == 0 and args.time_blocking == -1
     tb = args.time_blocking
     if tb == -1:
-        k32, k64 = "bulk2_kernel<float,float,10>", "bulk2_kernel<double,double,8>"
+        k32, k64 = "bulk2_kernel<float,float,10>", "bulk2_kernel<double,double,12>"
     elif tb == 0:
         k32, k64 = "bulk_kernel<float,float,false,304>", "bulk_kernel<double,double,false,8>"
     else:

@@ -64,7 +64,7 @@ typedef struct mrlbm_params {
     int32_t time_blocking;        /* 0: one bulk pass per step; 2..12 (even): two steps per bulk pass
                                      over the plain interior (strips of that many rows), the edges
                                      and the cylinder band in two ordinary sub-steps; -1 (default)
-                                     auto: 10 rows FP32, 8 rows FP64.  Same bits either way */
+                                     auto: 10 rows FP32, 12 rows FP64.  Same bits either way */
     int32_t reserved[4];
 } mrlbm_params;
 

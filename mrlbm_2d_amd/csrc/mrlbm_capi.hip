@@ -364,7 +364,7 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
     }
     {
         // -1 auto: the strip height measured fastest at 8192 x 2048 on MI355X
-        const int tb = p->time_blocking < 0 ? (h->f32 ? 10 : 8) : p->time_blocking;
+        const int tb = p->time_blocking < 0 ? (h->f32 ? 10 : 12) : p->time_blocking;
         if (tb < 0 || tb > 12 || tb % 2) return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4, 6, 8, 10 or 12");
         if (tb && h->nx >= 16 && h->ny >= 16) {
             const int rc3 = setup_time_blocking(h, tb);
