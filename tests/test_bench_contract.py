@@ -1,0 +1,39 @@
+"""bench.py's command-line contract and its CPU-side pieces (no GPU needed)."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, REPO)
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
+
+
+def test_roofline_fields_and_arithmetic():
+    r = dict(achieved_gbs=7000.0, frac=7000.0 / bench.HBM_PEAK_GBS, steps_per_launch=2,
+             bytes_per_step=805_306_368, launch_ms=0.23)
+    out = bench.roofline(r, "bulk2_kernel<float,float,10>", 949_914_039)
+    assert out["bound"] == "hbm" and out["unit"] == "GB/s" and out["peak"] == 8000.0
+    assert out["frac"] == pytest.approx(0.875)
+    assert out["algorithmic_bytes_per_launch"] == 2 * 805_306_368
+    assert out["traffic"] == 949_914_039
+    json.dumps(out)
+
+
+def test_cpu_baseline_on_a_small_budget():
+    """The oracle baseline on the bench's grid, one timed step: the JSON fields the bench
+    line carries."""
+    out = bench.cpu_baseline(max_steps=1, budget_s=1e-9)
+    assert out["unit"] == "MLUPS" and out["kind"] == "port" and out["value"] > 0
+    assert out["cores"] >= 1
+    assert "8192x2048" in out["sample"]
