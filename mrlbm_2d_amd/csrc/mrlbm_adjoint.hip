@@ -138,7 +138,7 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_prep_kern
     }
     if (t < a.ny) {
         T f[Q], m[6];
-        gather_generic(a, a.nx - 2, t, f);
+        gather_edge(a, a.nx - 2, t, f);
         F::moments(f, m);
         d.outm[3 * t] = F::rho_of(m[0]);
         d.outm[3 * t + 1] = m[1];
@@ -239,7 +239,7 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_local_ker
     if (q == 12 && !in) return;
     const long long n = (long long)j * a.pitch + i;
     T fp[Q];
-    gather_generic(a, i, j, fp);
+    gather_edge(a, i, j, fp);
     D ff[Q];
 #pragma unroll
     for (int p = 0; p < Q; p++) ff[p] = D(F::dev ? fp[p] + F::base(p) : fp[p], p == q ? (T)1.0 : (T)0.0);
@@ -347,22 +347,39 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_bulk_kern
     const int i = (int)(t % a.nx), j = (int)(t / a.nx);
     const long long ns_ = (long long)j * a.pitch + i; // this node
     const T om = a.om;
-    T acc[6] = {0, 0, 0, 0, 0, 0};
+    // Everything the 9 downstream neighbours contribute is loaded before any arithmetic (one
+    // memory round trip, then the special-row values); a neighbour outside the domain (its
+    // f_k is a stale value) reads this node instead and is skipped below.
+    T l[Q][6], rho[Q], ux[Q], uy[Q], chi[Q];
+    int row[Q];
+    bool inside[Q], bulk[Q];
+#pragma unroll
     for (int k = 0; k < Q; k++) {
         const int xi = i + EXk(k), xj = j + EYk(k);
-        if (xi < 0 || xi >= a.nx || xj < 0 || xj >= a.ny) continue; // f_k there is a stale value
-        const long long nx_ = (long long)xj * a.pitch + xi; // its downstream neighbour
+        inside[k] = xi >= 0 && xi < a.nx && xj >= 0 && xj < a.ny;
+        const int ci = inside[k] ? xi : i, cj = inside[k] ? xj : j;
+        const long long n = (long long)cj * a.pitch + ci;
+        bulk[k] = inside[k] && !on_ring(a, ci, cj) && !is_wall(a, ci, cj);
+        load_lam(a, d.lam1, n, l[k]);
+        rho[k] = F::rho_of((T)d.m1[n]);
+        ux[k] = (T)d.m1[a.plane + n];
+        uy[k] = (T)d.m1[2 * a.plane + n];
+        chi[k] = a.chi ? a.chi[n] : (T)0.0;
+        row[k] = d.spec[n];
+    }
+    T acc[6] = {0, 0, 0, 0, 0, 0};
+    T lchi0 = (T)0.0; // this node's own chi term (k = 0 is the node itself)
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        if (!inside[k]) continue;
         T ak = (T)0.0;
-        if (!on_ring(a, xi, xj) && !is_wall(a, xi, xj)) {
-            T l[6], lp[6], lchi;
-            load_lam(a, d.lam1, nx_, l);
-            const T rho = F::rho_of((T)d.m1[nx_]), ux = (T)d.m1[a.plane + nx_], uy = (T)d.m1[2 * a.plane + nx_];
-            const T chi = a.chi ? a.chi[nx_] : (T)0.0;
-            bulk_pre(l, rho, ux, uy, chi, om, lp, lchi);
+        if (bulk[k]) {
+            T lp[6], lchi;
+            bulk_pre(l[k], rho[k], ux[k], uy[k], chi[k], om, lp, lchi);
             ak = pop_adjoint(lp, k);
+            if (k == 0) lchi0 = lchi;
         }
-        const int row = d.spec[nx_];
-        if (row >= 0) ak = ak + d.E[row * Q + k];
+        if (row[k] >= 0) ak = ak + d.E[row[k] * Q + k];
         T v[6];
         recon_row<T>(k, v);
         const T wa = wk<T>(k) * ak;
@@ -371,13 +388,7 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_bulk_kern
     }
 #pragma unroll
     for (int p = 0; p < 6; p++) d.lam0[p * a.plane + ns_] = acc[p];
-    if (a.chi && !on_ring(a, i, j) && !is_wall(a, i, j)) {
-        T l[6], lp[6], lchi;
-        load_lam(a, d.lam1, ns_, l);
-        const T rho = F::rho_of((T)d.m1[ns_]), ux = (T)d.m1[a.plane + ns_], uy = (T)d.m1[2 * a.plane + ns_];
-        bulk_pre(l, rho, ux, uy, a.chi[ns_], om, lp, lchi);
-        d.G[ns_] += (double)lchi;
-    }
+    if (a.chi && bulk[0]) d.G[ns_] += (double)lchi0;
 }
 
 template <class S, class T> static int adjoint_step(const StepArgs<S, T> &a, const AdjArgs<S, T> &d, hipStream_t st)
