@@ -180,6 +180,13 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_local_ker
         const int i = b.ij[2 * k], j = b.ij[2 * k + 1];
         const long long n = (long long)j * a.pitch + i;
         const double *R = b.real + (long long)k * BT_NREAL;
+        // loads that do not depend on the arithmetic first: lam of the node, its special row,
+        // the interpolated wall moments' inputs, then the gather
+        T l[6];
+        load_lam(a, d.lam1, n, l);
+        const int srow = d.spec[n];
+        T Mxx0 = (T)0.0, Myy0 = (T)0.0;
+        if (b.rotated) wall_moments(b, (const T *)b.U, k, Mxx0, Myy0);
         T fp[Q];
         gather_interior(a, i, j, fp);
         D ff[Q];
@@ -191,8 +198,6 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_local_ker
         const uint16_t in = b.incs[lab], out = b.outs[lab];
         D m[6];
         if (b.rotated) {
-            T Mxx0, Myy0;
-            wall_moments(b, (const T *)b.U, k, Mxx0, Myy0);
             const D Mxx(Mxx0, q == 11 ? (T)1.0 : (T)0.0), Myy(Myy0, q == 12 ? (T)1.0 : (T)0.0);
             const D c = (D)R[BT_CTH], s = (D)R[BT_STH];
             const D uxp = ux * c + uy * s, uyp = uy * c - ux * s;
@@ -206,8 +211,6 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_local_ker
         collide<D>(m, D(om), D(om1), a.incomp_bulk);
         D o[6];
         conserved(m, o);
-        T l[6];
-        load_lam(a, d.lam1, n, l);
         if (d.wJ != (T)0.0) { // drag of this step: - sum_out fout_q c_qx, fout = recon(post state)
             for (int qq = 0; qq < Q; qq++)
                 if (((out >> qq) & 1) && EXk(qq) != 0) {
@@ -223,7 +226,7 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_local_ker
         for (int p = 0; p < 6; p++) g = g + l[p] * o[p].d;
         if (q < Q) {
             if (d.wJ != (T)0.0 && ((in >> q) & 1)) g = g + d.wJ * (T)EXk(q); // + sum_in f_q c_qx
-            d.E[d.spec[n] * Q + q] = g;
+            d.E[srow * Q + q] = g;
         } else if (q < 11) {
             d.lub_own[2 * k + (q - 9)] = g;
         } else {
@@ -238,6 +241,9 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_local_ker
     if (q >= 9 && q <= 11 && !out) return;
     if (q == 12 && !in) return;
     const long long n = (long long)j * a.pitch + i;
+    T l[6];
+    load_lam(a, d.lam1, n, l);
+    const int srow = d.spec[n];
     T fp[Q];
     gather_edge(a, i, j, fp);
     D ff[Q];
@@ -262,12 +268,10 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_local_ker
     collide<D>(mm, D(om), D(om1), a.incomp_bulk);
     D o[6];
     conserved(mm, o);
-    T l[6];
-    load_lam(a, d.lam1, n, l);
     T g = (T)0.0;
 #pragma unroll
     for (int p = 0; p < 6; p++) g = g + l[p] * o[p].d;
-    if (q < Q) d.E[d.spec[n] * Q + q] = g;
+    if (q < Q) d.E[srow * Q + q] = g;
     else if (q < 12) d.lout[3 * j + (q - 9)] = g;
     else d.guin[j] += (double)g;
 }
