@@ -41,6 +41,7 @@ struct AdjointWork {
     double *guin = nullptr;    // dJ/d(uin_j), ny
     int32_t *spec = nullptr;   // row of each node in E, -1 if none (one plane)
     int ns = 0;
+    bool flat = false;         // one dependency level and no stencil node in column nx-2
     void *E = nullptr;         // ns * 9
     int nedge = 0;
     int32_t *edge_ij = nullptr;
@@ -187,13 +188,19 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_local_ker
         const int srow = d.spec[n];
         T Mxx0 = (T)0.0, Myy0 = (T)0.0;
         if (b.rotated) wall_moments(b, (const T *)b.U, k, Mxx0, Myy0);
+        T ux0, uy0; // the wall velocity: interpolated here with one dependency level, else by adj_wallvel
+        if (b.nlevels <= 1) {
+            wall_velocity_one(b, (const T *)b.U, k, ux0, uy0);
+        } else {
+            ux0 = ((const T *)b.ub)[2 * k];
+            uy0 = ((const T *)b.ub)[2 * k + 1];
+        }
         T fp[Q];
         gather_interior(a, i, j, fp);
         D ff[Q];
 #pragma unroll
         for (int p = 0; p < Q; p++) ff[p] = D(F::dev ? fp[p] + F::base(p) : fp[p], p == q ? (T)1.0 : (T)0.0);
-        const T *ub = (const T *)b.ub;
-        const D ux(ub[2 * k], q == 9 ? (T)1.0 : (T)0.0), uy(ub[2 * k + 1], q == 10 ? (T)1.0 : (T)0.0);
+        const D ux(ux0, q == 9 ? (T)1.0 : (T)0.0), uy(uy0, q == 10 ? (T)1.0 : (T)0.0);
         const int lab = b.label[k];
         const uint16_t in = b.incs[lab], out = b.outs[lab];
         D m[6];
@@ -342,6 +349,57 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_interp_ke
     }
 }
 
+// The same for walls of one dependency level with no stencil node in column nx-2 (the two
+// phases then write disjoint special rows): lub = lub_own, one thread per stencil node and
+// per outlet row across the grid.
+template <class S, class T> __global__ void __launch_bounds__(256) adj_interp_flat_kernel(StepArgs<S, T> a, AdjArgs<S, T> d)
+{
+    const BoundaryDev &b = a.b;
+    const bool interp_u = b.vel_interp || !b.rotated;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < b.nU) {
+        const int u = t, nU = b.nU;
+        const T *Uf = d.Ufull;
+        T lux = 0, luy = 0, lMx = 0, lMy = 0;
+        for (int e = d.rev_u_start[u]; e < d.rev_u_start[u + 1]; e++) {
+            const int kk = d.rev_u[e] >> 3, c = d.rev_u[e] & 7;
+            const T w = interp_weight<T>(b.real + (long long)kk * BT_NREAL, c);
+            if (interp_u) {
+                lux = lux + w * d.lub_own[2 * kk];
+                luy = luy + w * d.lub_own[2 * kk + 1];
+            }
+            if (b.rotated && b.mom_interp) {
+                lMx = lMx + w * d.lMi[2 * kk];
+                lMy = lMy + w * d.lMi[2 * kk + 1];
+            }
+        }
+        if (!interp_u)
+            for (int kk = 0; kk < b.nbct; kk++)
+                if (b.uself[kk] == u) {
+                    lux = lux + d.lub_own[2 * kk];
+                    luy = luy + d.lub_own[2 * kk + 1];
+                }
+        const T c2 = (T)b.urot[3 * u], s2 = (T)b.urot[3 * u + 1], sc = (T)b.urot[3 * u + 2];
+        const T lm3 = c2 * lMx + s2 * lMy, lm4 = s2 * lMx + c2 * lMy, lm5 = sc * (lMx - lMy);
+        const T rho = Uf[u], ux = Uf[nU + u], uy = Uf[2 * nU + u];
+        const T mxx = Uf[3 * nU + u], myy = Uf[4 * nU + u], mxy = Uf[5 * nU + u];
+        const int row = d.spec[b.unode[u]];
+        for (int q = 0; q < Q; q++) {
+            const T v = ((((lux * ((T)EXk(q) - ux) + luy * ((T)EYk(q) - uy)) + lm3 * (Hxxk<T>(q) - mxx))
+                          + lm4 * (Hyyk<T>(q) - myy)) + lm5 * (Hxyk<T>(q) - mxy)) / rho;
+            d.E[row * Q + q] = d.E[row * Q + q] + v;
+        }
+        return;
+    }
+    const int j = t - b.nU;
+    if (j >= a.ny) return;
+    const T lr = d.lout[3 * j], lx = d.lout[3 * j + 1], ly = d.lout[3 * j + 2];
+    const T rho = d.outm[3 * j], ux = d.outm[3 * j + 1], uy = d.outm[3 * j + 2];
+    const int row = d.spec[(long long)j * a.pitch + (a.nx - 2)];
+    for (int q = 0; q < Q; q++)
+        d.E[row * Q + q] = d.E[row * Q + q] + (lr + (lx * ((T)EXk(q) - ux) + ly * ((T)EYk(q) - uy)) / rho);
+}
+
 // Reverse step of every node: gather of the downstream population adjoints, plus dJ/dchi.
 template <class S, class T> __global__ void __launch_bounds__(256) adj_bulk_kernel(StepArgs<S, T> a, AdjArgs<S, T> d)
 {
@@ -395,17 +453,21 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_bulk_kern
     if (a.chi && bulk[0]) d.G[ns_] += (double)lchi0;
 }
 
-template <class S, class T> static int adjoint_step(const StepArgs<S, T> &a, const AdjArgs<S, T> &d, hipStream_t st)
+template <class S, class T>
+static int adjoint_step(const StepArgs<S, T> &a, const AdjArgs<S, T> &d, bool flat, hipStream_t st)
 {
     const BoundaryDev &b = a.b;
     hipError_t e = launch_stencil_moments<S, T>(a, st);
     if (e != hipSuccess) return MRLBM_E_HIP;
-    hipLaunchKernelGGL((adj_wallvel_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
+    if (b.nlevels > 1) hipLaunchKernelGGL((adj_wallvel_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
     const int nprep = std::max(std::max(d.ns * Q, b.nU), a.ny);
     hipLaunchKernelGGL((adj_prep_kernel<S, T>), dim3((nprep + 255) / 256), dim3(256), 0, st, a, d);
     const int nloc = (b.nbct + d.nedge) * 13;
     hipLaunchKernelGGL((adj_local_kernel<S, T>), dim3((nloc + 255) / 256), dim3(256), 0, st, a, d);
-    hipLaunchKernelGGL((adj_interp_kernel<S, T>), dim3(1), dim3(256), 0, st, a, d);
+    if (flat)
+        hipLaunchKernelGGL((adj_interp_flat_kernel<S, T>), dim3((b.nU + a.ny + 255) / 256), dim3(256), 0, st, a, d);
+    else
+        hipLaunchKernelGGL((adj_interp_kernel<S, T>), dim3(1), dim3(256), 0, st, a, d);
     const long long N = (long long)a.nx * a.ny;
     hipLaunchKernelGGL((adj_bulk_kernel<S, T>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, a, d);
     return hipGetLastError() == hipSuccess ? 0 : MRLBM_E_HIP;
@@ -500,6 +562,9 @@ template <class S, class T> static int adjoint_setup(mrlbm_handle *h, int K, int
     for (int e2 = 0; e2 < w->nedge; e2++) add((long long)edge[2 * e2 + 1] * pitch + edge[2 * e2]);
     for (int j = 0; j < ny; j++) add((long long)j * pitch + (nx - 2));
     w->ns = ns;
+    w->flat = bt.nlevels <= 1;
+    for (int u = 0; u < bt.nU; u++)
+        if (bt.uij[2 * u] == nx - 2) w->flat = false;
     int rc = upload_vec(h, w, spec, &w->spec);
     rc |= upload_vec(h, w, edge, &w->edge_ij);
     // reverse lists of the interpolation corners: entries k*8 + c
@@ -667,7 +732,7 @@ static int adjoint_typed(mrlbm_handle *h, const mrlbm_adjoint_opts *o, double *d
             d.rev_u_start = w->rev_u_start;
             d.rev_u = w->rev_u;
             d.wJ = step >= nsteps - window ? wJ : (T)0.0;
-            const int rc = adjoint_step<S, T>(a, d, h->st);
+            const int rc = adjoint_step<S, T>(a, d, w->flat, h->st);
             if (rc) FAIL(rc, "adjoint kernel launch failed");
             lc ^= 1;
         }

@@ -412,6 +412,28 @@ template <class T> __device__ __forceinline__ T wall_interp(const double *R, con
     return (t1 * (T)0.0 + t2 * bilin(xd2, yd2, c[0], c[1], c[2], c[3])) - t3 * bilin(xd3, yd3, c[4], c[5], c[6], c[7]);
 }
 
+// Velocity of boundary node k when no corner is another boundary node (one dependency
+// level): interpolated from the stencil nodes' u, or the node's own u without velocity
+// interpolation in the rotated BC (2d_mrlbm.f90:768-819).
+template <class T> __device__ __forceinline__ void wall_velocity_one(const BoundaryDev &b, const T *U, int k, T &vx, T &vy)
+{
+    const int nU = b.nU;
+    if (b.vel_interp || !b.rotated) {
+        const double *R = b.real + (long long)k * BT_NREAL;
+        T cx[8], cy[8];
+        for (int c = 0; c < 8; c++) {
+            const int u = b.uc[12 * k + c];
+            cx[c] = U[nU + u];
+            cy[c] = U[2 * nU + u];
+        }
+        vx = wall_interp(R, cx);
+        vy = wall_interp(R, cy);
+    } else {
+        vx = U[nU + b.uself[k]];
+        vy = U[2 * nU + b.uself[k]];
+    }
+}
+
 template <class S, class T> __device__ void wall_velocities(const StepArgs<S, T> &a)
 {
     const BoundaryDev &b = a.b;
