@@ -202,3 +202,54 @@ def test_adjoint_512x128_case():
     assert abs(ad - fd) <= 1e-4 * abs(fd)
     assert abs(r32.J - r64.J) <= 1e-4 * abs(r64.J)
     assert rel < 1e-2
+
+
+def test_cylinder_next_to_the_outlet_column():
+    """A wall whose interpolation stencil reaches column nx-2, where the outlet copies its
+    values (the two share special rows, so the transposed interpolation runs in one
+    workgroup): the directional derivative along smooth chi bumps and dJ/duo against
+    central differences of the forward run."""
+    nx, ny, nsteps, window, K = 48, 24, 200, 100, 48
+
+    def par(uo=UO, omega=0.0):
+        p = capi.default_params()
+        p.nx, p.ny = nx, ny
+        p.x_periodic, p.y_periodic = 0, 0
+        p.channel_with_cylinder = 1
+        p.x_c, p.y_c, p.r_c = 39.0, 12.5, 3.0
+        p.label_policy = capi.LABEL_FLUID  # this small wall has labels the reference leaves unset
+        p.Re, p.uo, p.omega = 20.0, uo, omega
+        p.inlet_profile = capi.INLET_PARABOLIC
+        p.precision = capi.MRLBM_FP64
+        return p
+
+    def objective(p, chi):
+        with capi.Lattice(p) as lat:
+            lat.set_brinkman(chi)
+            o = capi.default_step_opts()
+            o.post_process = 1
+            o.statsbegin = 2 + nsteps - window
+            o.statsend = 1 + nsteps
+            lat.step(2, nsteps, o)
+            _, fd, _ = lat.forces()
+            uref = (2.0 / 3.0) * p.uo
+            return float(np.sum(fd) / window / (0.5 * uref * uref * 2 * lat.info.r_cyl))
+
+    x = np.arange(1, nx + 1)[:, None]
+    y = np.arange(1, ny + 1)[None, :]
+    chi = 0.01 * np.exp(-((x - 33.0) ** 2 + (y - 15.0) ** 2) / 8.0)
+    with capi.Lattice(par()) as lat:
+        assert lat.info.n_stencil > 0
+        lat.set_brinkman(chi)
+        res, g = lat.adjoint_drag(2, nsteps, window, K)
+        omega = lat.info.omega
+    print(f"[gpu] near-outlet adjoint: J {res.J:.10f} dJ/duo {res.dJ_duo:.6e}", flush=True)
+    direction = np.exp(-((x - 36.0) ** 2 + (y - 9.0) ** 2) / 6.0)
+    eps = 1e-6
+    fd = (objective(par(omega=omega), chi + eps * direction)
+          - objective(par(omega=omega), chi - eps * direction)) / (2 * eps)
+    ad = float(np.sum(g * direction))
+    assert abs(ad - fd) <= 1e-5 * abs(fd) + 1e-12
+    h = 1e-6
+    fdu = (objective(par(uo=UO + h, omega=omega), chi) - objective(par(uo=UO - h, omega=omega), chi)) / (2 * h)
+    assert abs(res.dJ_duo - fdu) <= 1e-5 * abs(fdu)
