@@ -189,3 +189,35 @@ def test_large_grid_matches_oracle(oracle_mod):
         atol = 1e-15 * o.nbct
         np.testing.assert_allclose(gfd, fd, rtol=1e-12, atol=atol)
         np.testing.assert_allclose(gfl, fl, rtol=1e-12, atol=atol)
+
+
+def test_400x100_channel_20000_steps_matches_oracle(oracle_mod):
+    """The CPU-runnable channel at its full length: 400 x 100, cylinder r = 10 (the reference's
+    geometry with ncy = 20, L_front/back/top/bot = 3/16/2/2: centre (71, 50)), Re = 20,
+    uo = 0.05, from rest, 20000 FP64 steps with the drag/lift series over the last 10000:
+    fields bit for bit and every recorded force against the oracle."""
+    p = dict(gc.load("p1_walls")["params"])
+    p.update(ncy=20, L_front=3, L_back=16, L_top=2, L_bot=2, Re=20.0, uo=0.05, nx=0, ny=0,
+             iplot=100000, statsbegin=10002, statsend=200001, post_process=True)
+    o = oracle_mod.Oracle(gc.oracle_params(oracle_mod, p))
+    assert (o.nx, o.ny) == (400, 100)
+    with Lattice(params_from_namelist(p)) as lat:
+        print(f"[gpu] 400x100 x 20000 steps vs oracle ({oracle_mod.num_threads()} threads)", flush=True)
+        lat.step(2, 20000, step_opts_from_namelist(p))
+        rho, ux, uy = lat.fields()
+        it, gfd, gfl = lat.forces()
+    o.run(2, 10000)
+    fd, fl = [], []
+    for k in range(10002, 20002):
+        o.step(k)
+        a, b = o.forces()
+        fd.append(a)
+        fl.append(b)
+    orho, oux, ouy, *_ = o.moments()
+    np.testing.assert_array_equal(rho, orho)
+    np.testing.assert_array_equal(ux, oux)
+    np.testing.assert_array_equal(uy, ouy)
+    np.testing.assert_array_equal(it, np.arange(10002, 20002, dtype=np.int32))
+    scale = max(np.abs(fd).max(), np.abs(fl).max())
+    np.testing.assert_allclose(gfd, fd, rtol=1e-12, atol=1e-12 * scale)
+    np.testing.assert_allclose(gfl, fl, rtol=1e-12, atol=1e-12 * scale)
