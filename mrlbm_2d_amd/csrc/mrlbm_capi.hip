@@ -311,7 +311,10 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
     if (p->nx < 4 || p->ny < 4) return fail(MRLBM_E_ARG, "nx and ny must be >= 4");
     if (p->precision != MRLBM_FP64 && p->precision != MRLBM_FP32_DEV) return fail(MRLBM_E_ARG, "unknown precision");
     if (p->mass_bc < 0 || p->mass_bc > 2) return fail(MRLBM_E_ARG, "unknown mass_bc");
-    if ((long long)p->nx * p->ny > (1LL << 31) - 1) return fail(MRLBM_E_ARG, "grid too large for 32-bit node index");
+    // boundary tables and probes hold node indices i + pitch * j (pitch = nx rounded up to 64)
+    // as 32-bit integers
+    if (((long long)p->nx + 63) / 64 * 64 * p->ny > (1LL << 31) - 1)
+        return fail(MRLBM_E_ARG, "grid too large for 32-bit node index");
     {
         const int k = p->kernel;
         if (k != 0 && k != 1 && k != 2 && k != 4 && k != 8 && k != 16 && k != 304 && k != 308 && k != 316)

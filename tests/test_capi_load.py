@@ -57,6 +57,20 @@ def test_create_rejects_bad_bulk_options_without_gpu():
             raise AssertionError(f"{field}={value} accepted")
 
 
+def test_create_rejects_grids_beyond_32_bit_node_indices_without_gpu():
+    """Node indices i + pitch * j (pitch = nx rounded up to 64) are 32-bit: 46341 x 46340
+    nodes fit (2,147,441,940), but with the pitch 46400 x 46340 = 2,150,176,000 does not."""
+    assert 46341 * 46340 < 2**31 - 1 < 46400 * 46340
+    p = capi.default_params()
+    p.nx, p.ny = 46341, 46340
+    try:
+        capi.Lattice(p)
+    except capi.MrlbmError as e:
+        assert e.code == capi.E_ARG and "32-bit" in str(e), e
+    else:
+        raise AssertionError("grid accepted")
+
+
 def test_default_params_pick_the_measured_variants():
     p = capi.default_params()
     assert p.kernel == 0 and p.time_blocking == -1 and p.block_order == 0
