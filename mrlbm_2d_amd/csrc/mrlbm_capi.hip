@@ -381,8 +381,8 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
 extern "C" int mrlbm_destroy(mrlbm_handle *h)
 {
     if (!h) return MRLBM_E_ARG;
-    (void)hipStreamSynchronize(h->st);
-    (void)hipStreamSynchronize(h->st2);
+    for (hipStream_t q : {h->st, h->st2, h->st3})
+        if (q) (void)hipStreamSynchronize(q);
     free_all(h);
     delete h;
     return MRLBM_OK;
