@@ -61,7 +61,7 @@ typedef struct mrlbm_params {
                                      of that many rows with 62 written columns per wave, 304/308/316
                                      aligned 64-column waves with an unrolled strip of kernel-300 rows */
     int32_t block_order;          /* bulk blocks: 0 plain, 1 XCD-aware (vertical neighbours share an L2) */
-    int32_t time_blocking;        /* 0: one bulk pass per step; 2..12 (even): two steps per bulk pass
+    int32_t time_blocking;        /* 0: one bulk pass per step; 2..16 (even): two steps per bulk pass
                                      over the plain interior (strips of that many rows), the edges
                                      and the cylinder band in two ordinary sub-steps; -1 (default)
                                      auto: 10 rows FP32, 12 rows FP64.  Same bits either way */

@@ -320,8 +320,8 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
         if (k != 0 && k != 1 && k != 2 && k != 4 && k != 8 && k != 16 && k != 304 && k != 308 && k != 316)
             return fail(MRLBM_E_ARG, "kernel must be 0, 1, 2, 4, 8, 16, 304, 308 or 316");
         const int tb = p->time_blocking;
-        if (tb != -1 && tb != 0 && tb != 2 && tb != 4 && tb != 6 && tb != 8 && tb != 10 && tb != 12)
-            return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4, 6, 8, 10 or 12");
+        if (tb != -1 && (tb < 0 || tb > 16 || tb % 2))
+            return fail(MRLBM_E_ARG, "time_blocking must be -1 or an even number 0..16");
         if (p->strip_rows < 0 || p->strip_rows > 64) return fail(MRLBM_E_ARG, "strip_rows must be 0..64");
     }
     h->nx = p->nx;
@@ -368,7 +368,7 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
     {
         // -1 auto: the strip height measured fastest at 8192 x 2048 on MI355X
         const int tb = p->time_blocking < 0 ? (h->f32 ? 10 : 12) : p->time_blocking;
-        if (tb < 0 || tb > 12 || tb % 2) return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4, 6, 8, 10 or 12");
+        if (tb < 0 || tb > 16 || tb % 2) return fail(MRLBM_E_ARG, "time_blocking must be -1 or an even number 0..16");
         if (tb && h->nx >= 16 && h->ny >= 16) {
             const int rc3 = setup_time_blocking(h, tb);
             if (rc3) return fail(rc3, "");

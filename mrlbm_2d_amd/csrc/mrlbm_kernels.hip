@@ -1028,6 +1028,8 @@ hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t
     case 8: launch_timed(bulk2_kernel<S, T, 8>, g, b, st, t0, t1, a); break;
     case 10: launch_timed(bulk2_kernel<S, T, 10>, g, b, st, t0, t1, a); break;
     case 12: launch_timed(bulk2_kernel<S, T, 12>, g, b, st, t0, t1, a); break;
+    case 14: launch_timed(bulk2_kernel<S, T, 14>, g, b, st, t0, t1, a); break;
+    case 16: launch_timed(bulk2_kernel<S, T, 16>, g, b, st, t0, t1, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -44,7 +44,7 @@ def test_create_reports_bad_arguments_without_gpu():
 
 
 def test_create_rejects_bad_bulk_options_without_gpu():
-    for field, value in (("kernel", 3), ("kernel", 305), ("time_blocking", 3), ("time_blocking", 14), ("time_blocking", -2),
+    for field, value in (("kernel", 3), ("kernel", 305), ("time_blocking", 3), ("time_blocking", 18), ("time_blocking", -2),
                          ("strip_rows", 65), ("precision", 7), ("mass_bc", 5)):
         p = capi.default_params()
         p.nx, p.ny = 64, 32

@@ -78,7 +78,7 @@ def test_bulk_kernel_variants_agree(name, kernel):
         np.testing.assert_array_equal(uy, g["uy"])
 
 
-@pytest.mark.parametrize("time_blocking", [2, 6, 8, 12])
+@pytest.mark.parametrize("time_blocking", [2, 6, 8, 12, 14, 16])
 @pytest.mark.parametrize("name", ["p1_walls", "s_periodic", "s_xyper_noise", "chan"])
 def test_time_blocking_strips_agree(name, time_blocking):
     g, lat = run_golden(name, time_blocking=time_blocking)
