@@ -234,7 +234,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--time-blocking", type=int, default=-1,
-                    help="-1 auto (two steps per bulk pass), 0 one step per pass, 2..12 two-step strip rows")
+                    help="-1 auto (two steps per bulk pass), 0 one step per pass, 2..16 two-step strip rows")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (for profiler counter scaling)")
     args = ap.parse_args()

@@ -97,7 +97,7 @@ def test_fp32_variants_identical_and_close_to_reference(name):
     g = gc.load(name)
     p = g["params"]
     results = {}
-    for kernel in (0, 1, 2, 4, 8, 16, 304, 308, 316, 1002, 1004, 1006, 1010, 1012):
+    for kernel in (0, 1, 2, 4, 8, 16, 304, 308, 316, 1002, 1004, 1006, 1010, 1012, 1016):
         par = params_from_namelist(p, precision=capi.MRLBM_FP32_DEV)
         par.kernel = kernel if kernel < 1000 else 0
         par.time_blocking = kernel - 1000 if kernel >= 1000 else 0
