@@ -672,12 +672,11 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         }
     };
     // a pair of iterations may share one two-step bulk pass when neither needs per-node
-    // work of the bulk (plot-step zeroing, u sampling, probes) and the records fit
+    // work of the bulk (plot-step zeroing, u sampling) and the records fit
     auto pair_ok = [&](int iter) {
         if (!h->tb_H || h->from_pops || h->chi || h->p.kernel == 1) return false;
         for (int it = iter; it <= iter + 1; it++) {
             if (plot_step(it) || sample_step(it)) return false;
-            if (is_window(it) && opts.nprobe > 0) return false;
         }
         const int nrec = (is_window(iter) ? 1 : 0) + (is_window(iter + 1) ? 1 : 0);
         return (int)rec_iters.size() + nrec <= std::max(h->rec_cap, 0) || nrec == 0;
@@ -740,6 +739,14 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             HIPCHK(launch_bulk_pair(bulk));
             a1.nrect = h->nrect1;
             std::memcpy(a1.rect, h->rect1, sizeof h->rect1);
+            const int slot2 = is_window(iter + 1) ? slot1 + (is_window(iter) ? 1 : 0) : -1;
+            if (opts.nprobe > 0 && (a1.rec_slot >= 0 || slot2 >= 0)) {
+                // the probes the bulk carries; the next pair's side work (which overwrites
+                // state t's band nodes) then waits for this kernel too
+                HIPCHK(launch_pair_probes<S, T>(a1, (const S *)C, slot2, h->st));
+                HIPCHK(hipEventRecord(h->evA, h->st));
+                h->bulk_done = h->evA;
+            }
             HIPCHK(side_work(a1));
             after_iter(iter);
             auto a2 = step_args(iter + 1, h->S2, C, slot1 + (is_window(iter) ? 1 : 0));

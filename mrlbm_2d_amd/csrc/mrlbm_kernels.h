@@ -70,6 +70,10 @@ template <class S, class T>
 hipError_t launch_edges(const StepArgs<S, T> &a, hipStream_t st);
 template <class S, class T>
 hipError_t launch_inlet_fold(const StepArgs<S, T> &a, hipStream_t st);
+// Probes of a two-step pair on the nodes the two-step bulk carries (a: sub-step 1 with its
+// band rectangles; s2, slot2: state t+2 and its record slot, or -1).
+template <class S, class T>
+hipError_t launch_pair_probes(const StepArgs<S, T> &a, const S *s2, int slot2, hipStream_t st);
 template <class S, class T>
 hipError_t launch_get_populations(const StepArgs<S, T> &a, double *f_dense, hipStream_t st);
 template <class S, class T>

@@ -854,6 +854,46 @@ template <class S, class T> __global__ void __launch_bounds__(256) bulk_simple_k
     bulk_simple(a, blockIdx.x);
 }
 
+// Probes inside a two-step pair, one thread per probe.  The ring, band and wall kernels of
+// each sub-step record the probes on the nodes they write; this records the others, which
+// the two-step bulk carries through t+1 without storing them: at t+1 the moments gathered
+// from state t (what the bulk computes there; the collision leaves rho and u unchanged) for
+// a node that sub-step 1's side work does not write, and at t+2 the stored rho and u of a
+// node that the bulk writes.  a: sub-step 1 (src = state t, its band rectangles, the record
+// slot of t+1 or -1); s2, slot2: state t+2 and its record slot (or -1).
+template <class S, class T> __global__ void __launch_bounds__(64) pair_probes_kernel(StepArgs<S, T> a, const S *s2, int slot2)
+{
+    using F = typename FormOf<T>::F;
+    const int p = threadIdx.x;
+    if (p >= a.nprobe) return;
+    const long long n = a.probe_node[p];
+    const int c = (int)(n % a.pitch), j = (int)(n / a.pitch);
+    if (a.rec_slot >= 0) {
+        bool side = c == 0 || c == a.nx - 1 || j == 0 || j == a.ny - 1;
+        for (int q = 0; q < a.nrect; q++)
+            side = side || (c >= a.rect[q][0] && c < a.rect[q][0] + a.rect[q][2] && j >= a.rect[q][1] &&
+                            j < a.rect[q][1] + a.rect[q][3]);
+        const int bi = c - a.bx0, bj = j - a.by0;
+        if ((unsigned)bi < (unsigned)a.bw && (unsigned)bj < (unsigned)a.bh) side = side || (a.boxcls[bj * a.bw + bi] & 1);
+        if (!side) {
+            T f[Q], m[6];
+            gather_interior(a, c, j, f);
+            F::moments(f, m);
+            double *r = a.rec + (long long)a.rec_slot * a.rec_stride + 2 + 3 * p;
+            r[0] = F::dev ? 1.0 + (double)m[0] : (double)m[0];
+            r[1] = m[1];
+            r[2] = m[2];
+        }
+    }
+    if (slot2 >= 0 && two_step_node(a, c, j)) {
+        const S r0 = s2[n], u = s2[a.plane + n], v = s2[2 * a.plane + n];
+        double *r = a.rec + (long long)slot2 * a.rec_stride + 2 + 3 * p;
+        r[0] = F::dev ? 1.0 + (double)r0 : (double)r0;
+        r[1] = u;
+        r[2] = v;
+    }
+}
+
 template <class S, class T> __global__ void __launch_bounds__(256) inlet_fold_kernel(StepArgs<S, T> a)
 {
     __shared__ double sh[16];
@@ -1062,6 +1102,12 @@ template <class S, class T> hipError_t launch_stencil_moments(const StepArgs<S, 
     hipLaunchKernelGGL((stencil_moments_kernel<S, T>), dim3((a.b.nU + 63) / 64), dim3(64), 0, st, a);
     return hipGetLastError();
 }
+template <class S, class T>
+hipError_t launch_pair_probes(const StepArgs<S, T> &a, const S *s2, int slot2, hipStream_t st)
+{
+    hipLaunchKernelGGL((pair_probes_kernel<S, T>), dim3(1), dim3(64), 0, st, a, s2, slot2);
+    return hipGetLastError();
+}
 template <class S, class T> hipError_t launch_inlet_fold(const StepArgs<S, T> &a, hipStream_t st)
 {
     hipLaunchKernelGGL((inlet_fold_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
@@ -1100,6 +1146,7 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_bulk2_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, hipStream_t); \
     template int bulk_batch_resident_blocks<S, T>(int);                                          \
     template hipError_t launch_inlet_fold<S, T>(const StepArgs<S, T> &, hipStream_t);            \
+    template hipError_t launch_pair_probes<S, T>(const StepArgs<S, T> &, const S *, int, hipStream_t); \
     template hipError_t launch_get_populations<S, T>(const StepArgs<S, T> &, double *, hipStream_t); \
     template hipError_t launch_ghost_from_state<S, T>(const StepArgs<S, T> &, T *, hipStream_t);  \
     template hipError_t launch_ghost_from_pops<S, T>(const StepArgs<S, T> &, T *, hipStream_t);

@@ -39,7 +39,7 @@ def run_golden(name, kernel=0, time_blocking=0):
 @pytest.mark.parametrize("name", gc.names())
 def test_fp64_matches_reference(name, time_blocking):
     """Also with two steps per bulk pass: pairs of steps share one pass wherever no
-    plot-step zeroing or probe falls inside them, single steps elsewhere."""
+    plot-step zeroing or u sampling falls inside them, single steps elsewhere."""
     g, lat = run_golden(name, time_blocking=time_blocking)
     with lat:
         rho, ux, uy = lat.fields()
@@ -221,3 +221,43 @@ def test_400x100_channel_20000_steps_matches_oracle(oracle_mod):
     scale = max(np.abs(fd).max(), np.abs(fl).max())
     np.testing.assert_allclose(gfd, fd, rtol=1e-12, atol=1e-12 * scale)
     np.testing.assert_allclose(gfl, fl, rtol=1e-12, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("precision", [capi.MRLBM_FP64, capi.MRLBM_FP32_DEV])
+@pytest.mark.parametrize("name,probes", [
+    # the reference's probe points (2d_mrlbm.f90:104-118): in the cylinder box and the bulk
+    ("p1_walls", ((29, 21), (33, 21), (37, 21), (41, 21))),
+    # row 2 (band at t+1, bulk at t+2), column 1 (band), the outlet column (ring), bulk
+    ("p1_walls", ((60, 3), (2, 20), (88, 20), (60, 20))),
+    ("s_xyper_noise", ((40, 3), (2, 20), (56, 20), (45, 20))),
+])
+def test_probes_in_two_step_pairs_match_single_steps(name, probes, precision):
+    """Probe records inside two-step pairs (nodes the bulk carries through t+1 recorded by
+    their own kernel, the rest by the side kernels) equal those of single steps, bit for bit,
+    with the statistics window opening and closing inside a pair."""
+    g = gc.load(name)
+    p = dict(g["params"])
+    p.update(post_process=True, statsbegin=7, statsend=50)
+    out = {}
+    for tb in (0, -1):
+        par = params_from_namelist(p, precision=precision)
+        par.time_blocking = tb
+        with Lattice(par) as lat:
+            if "f0" in g:
+                lat.set_populations(g["f0"])
+            print(f"[gpu] probes {name} {probes} precision={precision} time_blocking={tb}", flush=True)
+            o = step_opts_from_namelist(p)
+            o.nprobe = len(probes)
+            for k, (i, j) in enumerate(probes):
+                o.probe_i[k], o.probe_j[k] = i, j
+            lat.set_timing(1)
+            lat.step(2, 60, o)
+            _, launches = lat.last_bulk_ms()
+            it, v = lat.probes()
+            out[tb] = (launches, it, v, lat.fields())
+    assert out[0][0] == 60 and out[-1][0] < 40, (out[0][0], out[-1][0])  # bulk launches
+    np.testing.assert_array_equal(out[-1][1], np.arange(7, 51, dtype=np.int32))
+    np.testing.assert_array_equal(out[-1][1], out[0][1])
+    np.testing.assert_array_equal(out[-1][2], out[0][2])
+    for a, b in zip(out[-1][3], out[0][3]):
+        np.testing.assert_array_equal(a, b)
