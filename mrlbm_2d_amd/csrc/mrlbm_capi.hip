@@ -174,6 +174,20 @@ template <class S, class T> static int create_typed(mrlbm_handle *h)
         b.fxy = dmalloc<double>((size_t)2 * t.nbct, e);
         HIPCHK(e);
         h->allocs.push_back(b.fxy);
+        // the surface-pressure smoothing weights (nbct^2 doubles; up to 1 GiB, else computed
+        // in every recording step)
+        if ((size_t)t.nbct * t.nbct * sizeof(double) <= ((size_t)1 << 30)) {
+            double *gw = dmalloc<double>((size_t)t.nbct * t.nbct, e);
+            HIPCHK(e);
+            h->allocs.push_back(gw);
+            double *gsw = dmalloc<double>(t.nbct, e);
+            HIPCHK(e);
+            h->allocs.push_back(gsw);
+            HIPCHK(launch_smooth_weights(b, gw, gsw, h->st));
+            HIPCHK(hipStreamSynchronize(h->st));
+            b.gw = gw;
+            b.gsw = gsw;
+        }
     }
     return init_rest_state<S, T>(h);
 }

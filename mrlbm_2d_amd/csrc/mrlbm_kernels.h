@@ -14,6 +14,8 @@ struct BoundaryDev {
     const double *urot;
     const int32_t *idx;
     const double *thetas;
+    const double *gw;  // surface-pressure smoothing weights of (t, q) at [q*nbct + t], or null
+    const double *gsw; // their sums over q (sequential), per t
     uint16_t incs[16], outs[16];
     double wfx[16], wfy[16]; // rest-population force per label (FP32 form only)
     int mass_bc, rotated, vel_interp, mom_interp;
@@ -48,6 +50,7 @@ template <class S, class T> struct StepArgs {
     int rec_slot;              // >= 0: record forces/probes/surface p into rec + slot*rec_stride
     double *rec;
     int rec_stride, record_p;
+    int smooth_apart;          // records: the surface-pressure smoothing runs in smooth_kernel
     int nprobe, probe_node[4];
     double *inlet_out;         // post-BC rho at i = 1 (ny), or null
     const double *inlet_prev;  // previous window step's inlet rho, or null
@@ -70,6 +73,9 @@ template <class S, class T>
 hipError_t launch_edges(const StepArgs<S, T> &a, hipStream_t st);
 template <class S, class T>
 hipError_t launch_inlet_fold(const StepArgs<S, T> &a, hipStream_t st);
+// The surface-pressure smoothing weights and their sums, once per handle (they depend on
+// the wall angles only).
+hipError_t launch_smooth_weights(const BoundaryDev &b, double *w, double *sw, hipStream_t st);
 // Probes of a two-step pair on the nodes the two-step bulk carries (a: sub-step 1 with its
 // band rectangles; s2, slot2: state t+2 and its record slot, or -1).
 template <class S, class T>

@@ -681,6 +681,62 @@ template <class S, class T, bool ONE_PASS> __device__ __forceinline__ void wall_
     }
 }
 
+// Weight of wall node q in the smoothed surface pressure at node t (2d_mrlbm.f90:722-741):
+// a Gaussian in the angle with sigma = 0.1.
+__device__ __forceinline__ double smooth_weight(double th_t, double th_q)
+{
+    const double den = 2.0 * (0.1 * 0.1);
+    const double d = th_t - th_q;
+    return exp((-(d * d)) / den);
+}
+
+// The weights depend on the wall angles only: computed once, stored column-major (q*n + t,
+// so that the per-step sums read them coalesced), with their sums in the reference's order.
+__global__ void __launch_bounds__(256) smooth_weights_kernel(BoundaryDev b, double *w, double *sw)
+{
+    const int n = b.nbct;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int q = 0; q < n; q++) {
+            const double v = smooth_weight(b.thetas[t], b.thetas[q]);
+            w[(long long)q * n + t] = v;
+            s = s + v;
+        }
+        sw[t] = s;
+    }
+}
+
+// Gaussian-smoothed surface pressure at sorted wall node t (2d_mrlbm.f90:722-741), one wave:
+// lane l sums the terms q = l, l+64, ... in order, then a fixed shuffle tree adds the 64
+// partial sums, so the value does not depend on which wave computes it.  The weight sum is
+// the precomputed one (the reference's sequential order) when the weights are stored.
+// The reference sums all nbct terms in order; this differs from it by rounding only.
+template <class S, class T> __device__ void smooth_output(const StepArgs<S, T> &a, double *rec, int t)
+{
+    const BoundaryDev &b = a.b;
+    const int lane = threadIdx.x & 63, n = b.nbct;
+    double sv = 0.0, sw = 0.0;
+    for (int q = lane; q < n; q += 64) {
+        const double w = b.gw ? b.gw[(long long)q * n + t] : smooth_weight(b.thetas[t], b.thetas[q]);
+        sv = sv + w * b.pcy[b.idx[q]];
+        sw = sw + w;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sv += __shfl_down(sv, o, 64);
+        sw += __shfl_down(sw, o, 64);
+    }
+    if (lane == 0) rec[14 + n + t] = sv / (b.gsw ? b.gsw[t] : sw);
+}
+
+// The smoothing of one recording step spread over single-wave workgroups (one output per
+// wave at a time): in one workgroup it read the nbct^2 weights with a few waves' memory
+// parallelism and took longer than the rest of the step.
+template <class S, class T> __global__ void __launch_bounds__(64) smooth_kernel(StepArgs<S, T> a)
+{
+    double *rec = a.rec + (long long)a.rec_slot * a.rec_stride;
+    for (int t = blockIdx.x; t < a.b.nbct; t += gridDim.x) smooth_output(a, rec, t);
+}
+
 // Net force, records and the inlet-density running mean of one step (one workgroup), after
 // every boundary node of the step has been written.
 template <class S, class T> __device__ void wall_records(const StepArgs<S, T> &a)
@@ -707,22 +763,11 @@ template <class S, class T> __device__ void wall_records(const StepArgs<S, T> &a
                 a.facc[3] += 1.0;
             }
         }
-        if (a.record_p) { // theta-sorted (:602-604) and Gaussian-smoothed (:722-741)
-            double *ps = rec + 14, *pf = rec + 14 + b.nbct;
+        if (a.record_p) { // theta-sorted (:602-604); smoothed here unless a launch of its own does it
+            double *ps = rec + 14;
             for (int t = tid; t < b.nbct; t += nt) ps[t] = b.pcy[b.idx[t]];
-            __syncthreads();
-            const double den = 2.0 * (0.1 * 0.1);
-            for (int t = tid; t < b.nbct; t += nt) {
-                double sw = 0.0, sv = 0.0;
-                const double th = b.thetas[t];
-                for (int q = 0; q < b.nbct; q++) {
-                    const double d = th - b.thetas[q];
-                    const double w = exp((-(d * d)) / den);
-                    sw = sw + w;
-                    sv = sv + w * b.pcy[b.idx[q]];
-                }
-                pf[t] = sv / sw;
-            }
+            if (!a.smooth_apart)
+                for (int t = tid >> 6; t < b.nbct; t += nt >> 6) smooth_output(a, rec, t);
         }
     }
     if (a.inlet_prev) { // running mean of the inlet density of the previous step (:357-360)
@@ -1089,8 +1134,13 @@ template <class S, class T> hipError_t launch_wall(const StepArgs<S, T> &a, hipS
     if (a.b.nlevels <= 1) {
         if (a.b.nbct > 0)
             hipLaunchKernelGGL((wall_nodes_kernel<S, T>), dim3((a.b.nbct + 63) / 64), dim3(64), 0, st, a);
-        if (a.rec_slot >= 0 || a.inlet_prev)
-            hipLaunchKernelGGL((wall_records_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
+        if (a.rec_slot >= 0 || a.inlet_prev) {
+            StepArgs<S, T> r = a;
+            r.smooth_apart = a.record_p && a.rec_slot >= 0 ? 1 : 0;
+            hipLaunchKernelGGL((wall_records_kernel<S, T>), dim3(1), dim3(256), 0, st, r);
+            if (r.smooth_apart)
+                hipLaunchKernelGGL((smooth_kernel<S, T>), dim3(std::min(a.b.nbct, 1024)), dim3(64), 0, st, r);
+        }
         return hipGetLastError();
     }
     const int threads = std::min(512, std::max(256, (a.b.nbct + 63) / 64 * 64));
@@ -1102,6 +1152,12 @@ template <class S, class T> hipError_t launch_stencil_moments(const StepArgs<S, 
     hipLaunchKernelGGL((stencil_moments_kernel<S, T>), dim3((a.b.nU + 63) / 64), dim3(64), 0, st, a);
     return hipGetLastError();
 }
+hipError_t launch_smooth_weights(const BoundaryDev &b, double *w, double *sw, hipStream_t st)
+{
+    hipLaunchKernelGGL(smooth_weights_kernel, dim3((b.nbct + 255) / 256), dim3(256), 0, st, b, w, sw);
+    return hipGetLastError();
+}
+
 template <class S, class T>
 hipError_t launch_pair_probes(const StepArgs<S, T> &a, const S *s2, int slot2, hipStream_t st)
 {
