@@ -205,9 +205,9 @@ static void free_all(mrlbm_handle *h)
     if (h->e1) (void)hipEventDestroy(h->e1);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     h->tev.clear();
-    for (hipEvent_t e : {h->evA, h->evB, h->evS, h->evW, h->evK})
+    for (hipEvent_t e : {h->evA, h->evB, h->evK})
         if (e) (void)hipEventDestroy(e);
-    for (hipStream_t q : {h->st, h->st2, h->st3})
+    for (hipStream_t q : {h->st, h->st2})
         if (q) (void)hipStreamDestroy(q);
 }
 
@@ -357,17 +357,14 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
     else h->nu = uref * (double)(p->ny - 1) / p->Re;
     h->omega = p->omega > 0 ? p->omega : 1.0 / ((3.0 * h->nu) + (1.0 / 2.0));
     if (p->omega > 0) h->nu = (1.0 / p->omega - 0.5) / 3.0;
-    if (hipStreamCreate(&h->st) != hipSuccess || hipStreamCreate(&h->st2) != hipSuccess ||
-        hipStreamCreate(&h->st3) != hipSuccess)
+    if (hipStreamCreate(&h->st) != hipSuccess || hipStreamCreate(&h->st2) != hipSuccess)
         return fail(MRLBM_E_HIP, "hipStreamCreate failed");
     // the stream-ordering events order work on this device only: a device-scope release
     // (no system-scope cache writeback each time one is recorded)
     const unsigned order_flags = hipEventDisableTiming | hipEventReleaseToDevice;
     if (hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess ||
         hipEventCreateWithFlags(&h->evA, order_flags) != hipSuccess || hipEventCreate(&h->evK) != hipSuccess ||
-        hipEventCreateWithFlags(&h->evB, order_flags) != hipSuccess ||
-        hipEventCreateWithFlags(&h->evS, order_flags) != hipSuccess ||
-        hipEventCreateWithFlags(&h->evW, order_flags) != hipSuccess)
+        hipEventCreateWithFlags(&h->evB, order_flags) != hipSuccess)
         return fail(MRLBM_E_HIP, "hipEventCreate failed");
     h->bulk_done = h->evA;
     if (hipEventRecord(h->evA, h->st) != hipSuccess || hipEventRecord(h->evB, h->st2) != hipSuccess)
@@ -395,7 +392,7 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
 extern "C" int mrlbm_destroy(mrlbm_handle *h)
 {
     if (!h) return MRLBM_E_ARG;
-    for (hipStream_t q : {h->st, h->st2, h->st3})
+    for (hipStream_t q : {h->st, h->st2})
         if (q) (void)hipStreamSynchronize(q);
     free_all(h);
     delete h;
@@ -634,40 +631,21 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         a.inlet_prev_c = h->prev_inlet_c;
         return a;
     };
-    // Side work of one iteration on stream B: the ring and band kernel there, and in
-    // parallel on stream C the curved wall (stencil nodes, then the wall nodes and, on
-    // recording steps, the records) or, without a cylinder, the inlet-density fold; stream B
-    // then waits for stream C.
-    // Small grids are launch-bound: there the wall chain stays on stream B after the ring and
-    // band kernel (which then also does the stencil nodes), saving the fork and join.
-    const bool split_wall = (long long)h->nx * h->ny >= (1LL << 22);
+    // Side work of one iteration on stream B: the ring, band and stencil-node kernel, then the
+    // curved wall (wall nodes and, on recording steps, the records) or, without a cylinder,
+    // the inlet-density fold.  The wall chain used to run on a third stream beside the ring
+    // kernel; every launch beside the bulk costs it time, and the fork and join with their
+    // extra launch cost more than the overlap gained (8192 x 2048: 2.2% FP32, 1% FP64).
     auto side_work = [&](const StepArgs<S, T> &a) -> hipError_t {
-        if (!split_wall) {
-            hipError_t e = launch_side<S, T>(a, h->st2, true);
-            if (e != hipSuccess) return e;
-            if (h->has_b) return launch_wall<S, T>(a, h->st2);
-            if (h->prev_inlet_valid) {
-                auto f = a;
-                f.inlet_prev = h->d_inlet[h->prev_inlet_buf];
-                return launch_inlet_fold<S, T>(f, h->st2);
-            }
-            return hipSuccess;
-        }
-        hipError_t e = hipEventRecord(h->evS, h->st2);
-        if (e == hipSuccess) e = hipStreamWaitEvent(h->st3, h->evS, 0);
-        if (e == hipSuccess) e = launch_side<S, T>(a, h->st2, false);
+        hipError_t e = launch_side<S, T>(a, h->st2, true);
         if (e != hipSuccess) return e;
-        if (h->has_b) {
-            e = launch_stencil_moments<S, T>(a, h->st3);
-            if (e == hipSuccess) e = launch_wall<S, T>(a, h->st3);
-        } else if (h->prev_inlet_valid) {
+        if (h->has_b) return launch_wall<S, T>(a, h->st2);
+        if (h->prev_inlet_valid) {
             auto f = a;
             f.inlet_prev = h->d_inlet[h->prev_inlet_buf];
-            e = launch_inlet_fold<S, T>(f, h->st3);
+            return launch_inlet_fold<S, T>(f, h->st2);
         }
-        if (e == hipSuccess) e = hipEventRecord(h->evW, h->st3);
-        if (e == hipSuccess) e = hipStreamWaitEvent(h->st2, h->evW, 0);
-        return e;
+        return hipSuccess;
     };
     // inlet-density bookkeeping after iteration iter
     auto after_iter = [&](int iter) {

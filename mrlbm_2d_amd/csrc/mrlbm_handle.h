@@ -36,12 +36,10 @@ struct mrlbm_handle {
     bool has_b = false;
     double nu = 0, omega = 0;
     hipStream_t st = nullptr;   // bulk kernel and every host-facing operation
-    hipStream_t st2 = nullptr;  // ring and band kernels, concurrent with the bulk
-    hipStream_t st3 = nullptr;  // curved-wall kernels, concurrent with the ring and band kernels
+    hipStream_t st2 = nullptr;  // side work (ring, bands, stencil, wall), concurrent with the bulk
     hipEvent_t evA = nullptr, evB = nullptr; // end of the last bulk / side work
     hipEvent_t evK = nullptr;      // stop event given to untimed bulk launches
     hipEvent_t bulk_done = nullptr; // the last bulk launch's stop event (evA before any)
-    hipEvent_t evS = nullptr, evW = nullptr; // fork to and join from the wall stream
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int timing = 0; // 0 off, else time every timing-th bulk launch
     double last_ms = 0;

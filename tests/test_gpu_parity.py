@@ -151,8 +151,7 @@ def test_bulk_timing_counts_launches(time_blocking, every, launches):
 
 
 def test_large_grid_matches_oracle(oracle_mod):
-    """From 4M nodes up the curved wall runs on a stream of its own beside the ring and band
-    kernel (below that both share one stream).  FP64 at 4352 x 1088 (the reference's
+    """A grid of 4.7M nodes with the cylinder wall 64 nodes wide.  FP64 at 4352 x 1088 (the reference's
     L_front/back/top/bot = 8/55/8/7 with ncy = 68) against the oracle, from the rest state
     with seeded +-1e-3 relative noise on every population: a single step from populations,
     two two-step passes, a single step; fields bit for bit and the drag/lift series."""
