@@ -663,12 +663,12 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             rec_iters.clear();
         }
     };
-    // a pair of iterations may share one two-step bulk pass when neither needs per-node
-    // work of the bulk (plot-step zeroing, u sampling) and the records fit
+    // a pair of iterations may share one two-step bulk pass when neither zeroes the solid
+    // nodes' velocity (plot steps) and the records fit
     auto pair_ok = [&](int iter) {
         if (!h->tb_H || h->from_pops || h->chi || h->p.kernel == 1) return false;
         for (int it = iter; it <= iter + 1; it++) {
-            if (plot_step(it) || sample_step(it)) return false;
+            if (plot_step(it)) return false;
         }
         const int nrec = (is_window(iter) ? 1 : 0) + (is_window(iter + 1) ? 1 : 0);
         return (int)rec_iters.size() + nrec <= std::max(h->rec_cap, 0) || nrec == 0;
@@ -726,6 +726,9 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             bulk.nstrips = h->nstrips2;
             bulk.ncg = h->ncg2;
             bulk.ncb = h->ncb2;
+            bulk.sample2 = sample_step(iter + 1) ? 1 : 0; // (bulk.sample: of iter)
+            bulk.nrect = h->nrect1;                        // what sub-step 1 writes itself
+            std::memcpy(bulk.rect, h->rect1, sizeof h->rect1);
             HIPCHK(hipStreamWaitEvent(h->st2, h->bulk_done, 0));
             HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
             HIPCHK(launch_bulk_pair(bulk));
@@ -745,6 +748,10 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             a2.nrect = h->nrect2;
             std::memcpy(a2.rect, h->rect2, sizeof h->rect2);
             HIPCHK(side_work(a2));
+            if (a1.sample && bulk.sample2) { // after the bulk and sub-step 1 (see the kernel)
+                HIPCHK(hipStreamWaitEvent(h->st2, h->bulk_done, 0));
+                HIPCHK(launch_pair_samples<S, T>(a1, (const S *)C, h->st2));
+            }
             after_iter(iter + 1);
             HIPCHK(hipEventRecord(h->evB, h->st2));
             h->cur ^= 1;

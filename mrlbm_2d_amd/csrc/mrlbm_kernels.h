@@ -45,7 +45,8 @@ template <class S, class T> struct StepArgs {
                                // 304/308/316 aligned 64-column waves, unrolled strip of variant-300 rows
     int xcd_order;             // 1: remap blocks so that each XCD takes consecutive strips of a column
     // recording
-    int sample;                // accumulate mean/rms u
+    int sample;                // accumulate mean/rms u (two-step bulk: of t+1)
+    int sample2;               // two-step bulk: accumulate those of t+2
     double *acc;               // 5 planes: ux, uy, ux^2, uy^2, ux*uy
     int rec_slot;              // >= 0: record forces/probes/surface p into rec + slot*rec_stride
     double *rec;
@@ -80,6 +81,10 @@ hipError_t launch_smooth_weights(const BoundaryDev &b, double *w, double *sw, hi
 // band rectangles; s2, slot2: state t+2 and its record slot, or -1).
 template <class S, class T>
 hipError_t launch_pair_probes(const StepArgs<S, T> &a, const S *s2, int slot2, hipStream_t st);
+// t+2 velocity samples of a pair with both steps sampled, on the nodes of sub-step 1's band
+// rectangles that the two-step bulk writes (a: sub-step 1; s2: state t+2); after both.
+template <class S, class T>
+hipError_t launch_pair_samples(const StepArgs<S, T> &a, const S *s2, hipStream_t st);
 template <class S, class T>
 hipError_t launch_get_populations(const StepArgs<S, T> &a, double *f_dense, hipStream_t st);
 template <class S, class T>

@@ -332,7 +332,35 @@ __device__ __forceinline__ void pull_row(bool left, bool right, const T up_prev[
     f[7] = right ? in2 : s7;
 }
 
-template <class S, class T, int H> __device__ void bulk_march2(const StepArgs<S, T> &a, int bb)
+// u statistics of one node (mean and rms accumulators, FP64), as in node_update.
+template <class S, class T> __device__ __forceinline__ void sample_u(const StepArgs<S, T> &a, long long n, T u, T v)
+{
+    const double ux = u, uy = v;
+    a.acc[n] += ux;
+    a.acc[a.plane + n] += uy;
+    a.acc[2 * a.plane + n] += ux * ux;
+    a.acc[3 * a.plane + n] += uy * uy;
+    a.acc[4 * a.plane + n] += ux * uy;
+}
+
+// Whether sub-step 1 of a two-step pair writes node (c, j) itself: the outer ring, the band
+// rectangles a.rect (sub-step 1's) and the wall's boundary nodes.  Columns past nx (row
+// padding) count as such too: nobody samples them.
+template <class S, class T> __device__ __forceinline__ bool side_node1(const StepArgs<S, T> &a, int c, int j)
+{
+    bool side = c <= 0 || c >= a.nx - 1 || j <= 0 || j >= a.ny - 1;
+    for (int q = 0; q < a.nrect; q++)
+        side = side || (c >= a.rect[q][0] && c < a.rect[q][0] + a.rect[q][2] && j >= a.rect[q][1] &&
+                        j < a.rect[q][1] + a.rect[q][3]);
+    const int bi = c - a.bx0, bj = j - a.by0;
+    if ((unsigned)bi < (unsigned)a.bw && (unsigned)bj < (unsigned)a.bh) side = side || (a.boxcls[bj * a.bw + bi] & 1);
+    return side;
+}
+
+// SAMPLE: also accumulate the u statistics: at t+1 (a.sample) of the strip's own rows where
+// sub-step 1 does not write the node, at t+2 (a.sample2) of the nodes the pass writes (with
+// t+1 sampled as well, only those that sub-step 1 does not write).
+template <class S, class T, int H, bool SAMPLE> __device__ void bulk_march2(const StepArgs<S, T> &a, int bb)
 {
     using F = typename FormOf<T>::F;
     constexpr int NQ = H + 4;             // state-t rows j0-2 .. j0+H+1
@@ -428,6 +456,11 @@ template <class S, class T, int H> __device__ void bulk_march2(const StepArgs<S,
         T f[Q], m1[6], G9[Q];
         pull_row(left, right, up_prev, mid_cur, F9, in0, in1, in2, f);
         F::moments(f, m1);
+        if constexpr (SAMPLE) {
+            const int j1 = j0 - 1 + p;
+            if (a.sample && p >= 1 && p <= H && !side_node1(a, c, j1))
+                sample_u(a, (long long)j1 * a.pitch + c, m1[1], m1[2]);
+        }
         collide<T>(m1, a.om, a.om1, a.incomp_bulk);
         up_prev[0] = up_cur[0]; up_prev[1] = up_cur[1]; up_prev[2] = up_cur[2];
         up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
@@ -441,8 +474,12 @@ template <class S, class T, int H> __device__ void bulk_march2(const StepArgs<S,
             const int j = j0 + p - 2;
             if (two_step_node(a, c, j)) {
                 F::moments(g2, m2);
-                collide<T>(m2, a.om, a.om1, a.incomp_bulk);
                 const long long n = (long long)j * a.pitch + c;
+                // when t+1 is sampled too, sub-step 1's side kernel adds t+1 into the nodes
+                // it writes concurrently: those are left to pair_samples_kernel
+                if constexpr (SAMPLE)
+                    if (a.sample2 && !(a.sample && side_node1(a, c, j))) sample_u(a, n, m2[1], m2[2]);
+                collide<T>(m2, a.om, a.om1, a.incomp_bulk);
 #pragma unroll
                 for (int q = 0; q < 6; q++) a.dst[q * a.plane + n] = (S)m2[q];
             }
@@ -459,15 +496,26 @@ template <class S, class T, int H> __device__ void bulk_march2(const StepArgs<S,
     }
 }
 
-template <class S, class T, int H> __global__ void __launch_bounds__(256) bulk2_kernel(StepArgs<S, T> a)
+template <class S, class T, int H, bool SAMPLE = false>
+__global__ void __launch_bounds__(256) bulk2_kernel(StepArgs<S, T> a)
 {
-    bulk_march2<S, T, H>(a, blockIdx.x);
+    bulk_march2<S, T, H, SAMPLE>(a, blockIdx.x);
 }
 
 // The side stream's node work of one step in one launch: the outer ring (edge update),
 // the band rectangles of a two-step pair (plain update) and the curved wall's stencil
 // nodes (pre-collision moments for boundary_kernel).  All three read only the source
 // state and write disjoint nodes.
+// Whether node (i, j) lies in one of the band rectangles a.rect[0 .. q-1].
+template <class S, class T> __device__ __forceinline__ bool in_rects(const StepArgs<S, T> &a, int q, int i, int j)
+{
+    bool in = false;
+    for (int r = 0; r < q; r++)
+        in = in || (i >= a.rect[r][0] && i < a.rect[r][0] + a.rect[r][2] && j >= a.rect[r][1] &&
+                    j < a.rect[r][1] + a.rect[r][3]);
+    return in;
+}
+
 template <class S, class T> __device__ void side_body(const StepArgs<S, T> &a, long long t, int nring, int nband)
 {
     if (t < nring) {
@@ -480,6 +528,7 @@ template <class S, class T> __device__ void side_body(const StepArgs<S, T> &a, l
             const long long n = (long long)a.rect[q][2] * a.rect[q][3];
             if (t < n) { // every rectangle lies in [1, nx-2] x [1, ny-2]: all 8 neighbours inside
                 const int i = a.rect[q][0] + (int)(t % a.rect[q][2]), j = a.rect[q][1] + (int)(t / a.rect[q][2]);
+                if (in_rects(a, q, i, j)) return; // an earlier rectangle has it (the box near an edge)
                 T f[Q];
                 gather_interior(a, i, j, f);
                 node_update(a, i, j, f, false);
@@ -890,7 +939,7 @@ __global__ void __launch_bounds__(256) bulk2_batch_kernel(const StepArgs<S, T> *
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 tl = tiles[t];
         const StepArgs<S, T> a = args[tl.x];
-        bulk_march2<S, T, H>(a, tl.y);
+        bulk_march2<S, T, H, false>(a, tl.y);
     }
 }
 
@@ -936,6 +985,27 @@ template <class S, class T> __global__ void __launch_bounds__(64) pair_probes_ke
         r[0] = F::dev ? 1.0 + (double)r0 : (double)r0;
         r[1] = u;
         r[2] = v;
+    }
+}
+
+// The t+2 samples of a pair with both steps sampled on the nodes that sub-step 1 writes and
+// the two-step bulk writes at t+2 (the bands' rows and columns beyond the ones sub-step 2
+// redoes), after both: from the stored rho and u of state t+2 (s2).  a: sub-step 1, with its
+// band rectangles.
+template <class S, class T> __global__ void __launch_bounds__(256) pair_samples_kernel(StepArgs<S, T> a, const S *s2)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int q = 0; q < a.nrect; q++) {
+        const long long n = (long long)a.rect[q][2] * a.rect[q][3];
+        if (t < n) {
+            const int i = a.rect[q][0] + (int)(t % a.rect[q][2]), j = a.rect[q][1] + (int)(t / a.rect[q][2]);
+            if (two_step_node(a, i, j) && !in_rects(a, q, i, j)) {
+                const long long m = (long long)j * a.pitch + i;
+                sample_u(a, m, (T)s2[a.plane + m], (T)s2[2 * a.plane + m]);
+            }
+            return;
+        }
+        t -= n;
     }
 }
 
@@ -1106,17 +1176,24 @@ template <class S, class T>
 hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0, hipEvent_t t1)
 {
     const dim3 g(nblocks), b(256);
+    const bool smp = a.sample || a.sample2;
+#define MRLBM_BULK2(HH)                                                                    \
+    case HH:                                                                               \
+        if (smp) launch_timed(bulk2_kernel<S, T, HH, true>, g, b, st, t0, t1, a);          \
+        else launch_timed(bulk2_kernel<S, T, HH, false>, g, b, st, t0, t1, a);             \
+        break;
     switch (H) {
-    case 2: launch_timed(bulk2_kernel<S, T, 2>, g, b, st, t0, t1, a); break;
-    case 4: launch_timed(bulk2_kernel<S, T, 4>, g, b, st, t0, t1, a); break;
-    case 6: launch_timed(bulk2_kernel<S, T, 6>, g, b, st, t0, t1, a); break;
-    case 8: launch_timed(bulk2_kernel<S, T, 8>, g, b, st, t0, t1, a); break;
-    case 10: launch_timed(bulk2_kernel<S, T, 10>, g, b, st, t0, t1, a); break;
-    case 12: launch_timed(bulk2_kernel<S, T, 12>, g, b, st, t0, t1, a); break;
-    case 14: launch_timed(bulk2_kernel<S, T, 14>, g, b, st, t0, t1, a); break;
-    case 16: launch_timed(bulk2_kernel<S, T, 16>, g, b, st, t0, t1, a); break;
+        MRLBM_BULK2(2)
+        MRLBM_BULK2(4)
+        MRLBM_BULK2(6)
+        MRLBM_BULK2(8)
+        MRLBM_BULK2(10)
+        MRLBM_BULK2(12)
+        MRLBM_BULK2(14)
+        MRLBM_BULK2(16)
     default: return hipErrorInvalidValue;
     }
+#undef MRLBM_BULK2
     return hipGetLastError();
 }
 template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st, bool stencil)
@@ -1158,6 +1235,13 @@ hipError_t launch_smooth_weights(const BoundaryDev &b, double *w, double *sw, hi
     return hipGetLastError();
 }
 
+template <class S, class T> hipError_t launch_pair_samples(const StepArgs<S, T> &a, const S *s2, hipStream_t st)
+{
+    long long n = 0;
+    for (int q = 0; q < a.nrect; q++) n += (long long)a.rect[q][2] * a.rect[q][3];
+    if (n > 0) hipLaunchKernelGGL((pair_samples_kernel<S, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, s2);
+    return hipGetLastError();
+}
 template <class S, class T>
 hipError_t launch_pair_probes(const StepArgs<S, T> &a, const S *s2, int slot2, hipStream_t st)
 {
@@ -1203,6 +1287,7 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template int bulk_batch_resident_blocks<S, T>(int);                                          \
     template hipError_t launch_inlet_fold<S, T>(const StepArgs<S, T> &, hipStream_t);            \
     template hipError_t launch_pair_probes<S, T>(const StepArgs<S, T> &, const S *, int, hipStream_t); \
+    template hipError_t launch_pair_samples<S, T>(const StepArgs<S, T> &, const S *, hipStream_t); \
     template hipError_t launch_get_populations<S, T>(const StepArgs<S, T> &, double *, hipStream_t); \
     template hipError_t launch_ghost_from_state<S, T>(const StepArgs<S, T> &, T *, hipStream_t);  \
     template hipError_t launch_ghost_from_pops<S, T>(const StepArgs<S, T> &, T *, hipStream_t);

@@ -260,3 +260,44 @@ def test_probes_in_two_step_pairs_match_single_steps(name, probes, precision):
     np.testing.assert_array_equal(out[-1][2], out[0][2])
     for a, b in zip(out[-1][3], out[0][3]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [capi.MRLBM_FP64, capi.MRLBM_FP32_DEV])
+@pytest.mark.parametrize("interval", [1, 3])
+def test_u_statistics_in_two_step_pairs_match_single_steps(oracle_mod, precision, interval):
+    """Mean / rms velocity accumulators sampled inside two-step pairs (the bulk accumulates
+    the nodes it carries through t+1 and writes at t+2, the side kernels the rest) equal
+    those of single steps bit for bit; in FP64 the means also equal the oracle's fields
+    averaged over the same steps."""
+    g = gc.load("p1_walls")
+    p = dict(g["params"])
+    p.update(post_process=True, statsbegin=5, statsend=60)
+    out = {}
+    for tb in (0, -1):
+        par = params_from_namelist(p, precision=precision)
+        par.time_blocking = tb
+        with Lattice(par) as lat:
+            print(f"[gpu] u statistics precision={precision} interval={interval} time_blocking={tb}", flush=True)
+            o = step_opts_from_namelist(p)
+            o.stats_interval = interval
+            lat.set_timing(1)
+            lat.step(2, 70, o)
+            _, launches = lat.last_bulk_ms()
+            out[tb] = (launches, lat.stats(), lat.fields())
+    assert out[0][0] == 70 and out[-1][0] < 50, (out[0][0], out[-1][0])  # bulk launches
+    for a, b in zip(out[-1][1], out[0][1]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(out[-1][2], out[0][2]):
+        np.testing.assert_array_equal(a, b)
+    if precision == capi.MRLBM_FP64:
+        orc = oracle_mod.Oracle(gc.oracle_params(oracle_mod, p))
+        su, sv, n = 0.0, 0.0, 0
+        for it in range(2, 72):
+            orc.step(it)
+            if 5 <= it <= 60 and (it - 5) % interval == 0:
+                _, ux, uy, *_ = orc.moments()
+                su, sv, n = su + ux, sv + uy, n + 1
+        mean_ux, mean_uy, _, _, ns = out[-1][1]
+        assert ns == n
+        np.testing.assert_allclose(mean_ux, su / n, rtol=1e-13, atol=1e-17)
+        np.testing.assert_allclose(mean_uy, sv / n, rtol=1e-13, atol=1e-17)
