@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # steps per launch: FP32 strips of 10 rows, FP64 of 12), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1);
 # the summary and the commands are in profiles/bulk2_kernel_8192x2048_hbm_bytes.md
-TRAFFIC_BYTES = {"f32": 949_914_039, "f64": 2_288_934_108}
+TRAFFIC_BYTES = {"f32": 951_367_036, "f64": 2_289_709_604}
 NX, NY = 8192, 2048
 # HIP event pairs around every 4th bulk launch: the per-launch mean without the events' own
 # cost (about 2% of the step when recorded around every launch)
@@ -270,11 +270,11 @@ def main():
     default_shape = args.strip_rows == 0 and args.time_blocking == -1
     tb = args.time_blocking
     if tb == -1:
-        k32, k64 = "bulk2_kernel<float,float,10>", "bulk2_kernel<double,double,12>"
+        k32, k64 = "bulk2_kernel<float,float,10,false>", "bulk2_kernel<double,double,12,false>"
     elif tb == 0:
         k32, k64 = "bulk_kernel<float,float,false,304>", "bulk_kernel<double,double,false,8>"
     else:
-        k32, k64 = f"bulk2_kernel<float,float,{tb}>", f"bulk2_kernel<double,double,{tb}>"
+        k32, k64 = f"bulk2_kernel<float,float,{tb},false>", f"bulk2_kernel<double,double,{tb},false>"
     out = {
         "metric": "MLUPS of the D2Q9 regularized LBM step, FP32 deviation form (FP64 beside it)",
         "value": round(r32["mlups"], 1),

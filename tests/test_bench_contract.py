@@ -22,11 +22,11 @@ def test_more_than_one_gpu_is_refused_with_a_message():
 def test_roofline_fields_and_arithmetic():
     r = dict(achieved_gbs=7000.0, frac=7000.0 / bench.HBM_PEAK_GBS, steps_per_launch=2,
              bytes_per_step=805_306_368, launch_ms=0.23)
-    out = bench.roofline(r, "bulk2_kernel<float,float,10>", 949_914_039)
+    out = bench.roofline(r, "bulk2_kernel<float,float,10,false>", 951_367_036)
     assert out["bound"] == "hbm" and out["unit"] == "GB/s" and out["peak"] == 8000.0
     assert out["frac"] == pytest.approx(0.875)
     assert out["algorithmic_bytes_per_launch"] == 2 * 805_306_368
-    assert out["traffic"] == 949_914_039
+    assert out["traffic"] == 951_367_036
     json.dumps(out)
 
 
