@@ -102,7 +102,10 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
     };
     for (int r = 0; r < R; r++) {
         mrlbm_handle *h = b->runs[r];
-        const int ring = 2 * h->nx + 2 * (h->ny - 2), nU = h->has_b ? h->bt.nU : 0;
+        // side blocks: the ring's column tiles, then 64 items each (ring rows or the whole
+        // ring, bands, stencil nodes), as side_block counts them
+        const int nct = ring_column_tiles(h->nx, h->ny, false);
+        const int ring = 64 * nct + (nct ? 2 * h->nx : 2 * h->nx + 2 * (h->ny - 2)), nU = h->has_b ? h->bt.nU : 0;
         auto band = [](const int rc[5][4], int n) {
             int t = 0;
             for (int q = 0; q < n; q++) t += rc[q][2] * rc[q][3];

@@ -143,7 +143,7 @@ template <class S, class T> __device__ void gather_interior(const StepArgs<S, T>
 // Dirichlet values and the regularized edge BC, in the reference's order
 // outlet -> inlet -> top -> bottom with the last writer winning (2d_mrlbm.f90:247-351).
 template <class S, class T>
-__device__ void edge_update(const StepArgs<S, T> &a, int c, int j, const T fpop[Q], T m[6])
+__device__ void edge_update(const StepArgs<S, T> &a, int c, int j, const T fpop[Q], T m[6], const T *gout = nullptr)
 {
     using F = typename FormOf<T>::F;
     const bool out = !a.xper && c == a.nx - 1, in = !a.xper && c == 0;
@@ -159,7 +159,12 @@ __device__ void edge_update(const StepArgs<S, T> &a, int c, int j, const T fpop[
     mm[0] = F::rho_of(m[0]);
     if (out) { // rho, u copied from column nx-1 before any BC (2d_mrlbm.f90:249-253)
         T g[Q], mo[6];
-        gather_edge(a, a.nx - 2, j, g);
+        if (gout) { // the caller's gather at (nx-2, j)
+#pragma unroll
+            for (int k = 0; k < Q; k++) g[k] = gout[k];
+        } else {
+            gather_edge(a, a.nx - 2, j, g);
+        }
         F::moments(g, mo);
         d_keep = mo[0];
         mm[0] = F::rho_of(mo[0]);
@@ -186,7 +191,8 @@ __device__ void edge_update(const StepArgs<S, T> &a, int c, int j, const T fpop[
 // penalization u <- (1 - chi) u of non-edge nodes (CHI, not in the reference), solid-node
 // zeroing on plot steps (2d_mrlbm.f90:375-384), recording, collision, store.
 template <class S, class T, bool CHI = false>
-__device__ __forceinline__ void node_update(const StepArgs<S, T> &a, int c, int j, const T f[Q], bool edge)
+__device__ __forceinline__ void node_update(const StepArgs<S, T> &a, int c, int j, const T f[Q], bool edge,
+                                            const T *gout = nullptr)
 {
     using F = typename FormOf<T>::F;
     int cls = 0;
@@ -196,7 +202,7 @@ __device__ __forceinline__ void node_update(const StepArgs<S, T> &a, int c, int 
     const long long n = (long long)j * a.pitch + c;
     T m[6];
     F::moments(f, m);
-    if (edge) edge_update(a, c, j, f, m);
+    if (edge) edge_update(a, c, j, f, m, gout);
     if constexpr (CHI) {
         if (a.chi && !edge) {
             const T keep = (T)1.0 - a.chi[n];

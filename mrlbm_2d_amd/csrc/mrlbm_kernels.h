@@ -98,6 +98,13 @@ hipError_t launch_stencil_moments(const StepArgs<S, T> &a, hipStream_t st);
 template <class S, class T>
 hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0 = nullptr,
                         hipEvent_t t1 = nullptr);
+// Workgroups of the ring's inlet and outlet columns when they go by column tiles (64 rows
+// each, staged in LDS): grids at least 8 wide stepping from moments; 0 otherwise (the
+// columns are then ordinary ring items).
+__host__ __device__ constexpr int ring_column_tiles(int nx, int ny, bool from_pops)
+{
+    return (!from_pops && nx >= 8 && ny >= 3) ? 2 * ((ny - 2 + 63) / 64) : 0;
+}
 // Side work of one step: ring + band rectangles (+ the wall's stencil nodes when stencil),
 // and the boundary workgroup.
 template <class S, class T>

@@ -549,9 +549,78 @@ template <class S, class T> __device__ __forceinline__ int band_nodes(const Step
     return n;
 }
 
-template <class S, class T> __global__ void __launch_bounds__(256) side_kernel(StepArgs<S, T> a, int nring, int nband)
+// The inlet and outlet columns of the outer ring (rows 1 .. ny-2) in tiles of 64 rows, one
+// wave each.  Down a column every gather load touches a cache line per lane (the outlet
+// gathers twice: its own populations and column nx-2's, which it copies), so the wave first
+// stages the moments its gathers read in LDS: rows j0-1 .. j0+64 of columns 0..1 (inlet)
+// or nx-3 .. nx-1 (outlet).  Sources outside the domain follow gather_edge (periodic wrap
+// from memory, else the stale-population table); the arithmetic is edge_body's.
+template <class S, class T> __device__ void ring_column_tile(const StepArgs<S, T> &a, int side, int j0)
 {
-    side_body(a, (long long)blockIdx.x * blockDim.x + threadIdx.x, nring, nband);
+    constexpr int R = 66;
+    __shared__ S sm[6][R][3];
+    const int lane = threadIdx.x;
+    const int xb = side == 0 ? -1 : a.nx - 3; // column of slot 0 (the inlet's is outside)
+    for (int idx = lane; idx < R * 3; idx += 64) {
+        const int r = idx / 3, k = idx - 3 * r, x = xb + k;
+        if (x < 0) continue;
+        const int j = min(max(j0 - 1 + r, 0), a.ny - 1); // clamped rows are never read
+        const long long n = (long long)j * a.pitch + x;
+#pragma unroll
+        for (int p = 0; p < 6; p++) sm[p][r][k] = a.src[p * a.plane + n];
+    }
+    __syncthreads();
+    const int j = j0 + lane;
+    if (j > a.ny - 2) return;
+    auto gather = [&](int i, T f[Q]) { // gather_edge at (i, j), 1 <= j <= ny-2
+#pragma unroll
+        for (int k = 0; k < Q; k++) {
+            const int si = i - EXk(k), sj = j - EYk(k);
+            S m[6];
+            if (si < 0 || si >= a.nx) {
+                if (!a.xper) {
+                    f[k] = a.ghost[(si < 0 ? j : a.ny + j) * Q + k];
+                    continue;
+                }
+                load_moments(a, si < 0 ? a.nx - 1 : 0, j, m);
+            } else {
+#pragma unroll
+                for (int p = 0; p < 6; p++) m[p] = sm[p][sj - (j0 - 1)][si - xb];
+            }
+            T g[Q];
+            FormOf<T>::F::recon(m, g);
+            f[k] = g[k];
+        }
+    };
+    T f[Q];
+    if (side == 0) {
+        gather(0, f);
+        node_update(a, 0, j, f, true);
+    } else {
+        T g[Q];
+        gather(a.nx - 1, f);
+        gather(a.nx - 2, g);
+        node_update(a, a.nx - 1, j, f, true, g);
+    }
+}
+
+// Side workgroups of one step (single waves): the ring columns as tiles when the column path
+// applies (ring_column_tiles), then 64 items each of the ring rows (or the whole ring), the
+// band rectangles and the stencil nodes.
+template <class S, class T> __device__ void side_block(const StepArgs<S, T> &a, int blk, int nband)
+{
+    const int nct = ring_column_tiles(a.nx, a.ny, a.pops != nullptr);
+    if (blk < nct) {
+        ring_column_tile(a, blk & 1, 1 + (blk >> 1) * 64);
+        return;
+    }
+    const int nring = nct ? 2 * a.nx : 2 * a.nx + 2 * (a.ny - 2);
+    side_body(a, (long long)(blk - nct) * 64 + threadIdx.x, nring, nband);
+}
+
+template <class S, class T> __global__ void __launch_bounds__(64) side_kernel(StepArgs<S, T> a, int nband)
+{
+    side_block(a, blockIdx.x, nband);
 }
 
 // Bulk, one node per thread: the first step after mrlbm_set_populations (populations in
@@ -913,7 +982,7 @@ template <class S, class T> __global__ void __launch_bounds__(64) side_tiles_ker
 {
     const int2 t = tiles[blockIdx.x];
     const StepArgs<S, T> &a = args[t.x];
-    side_body(a, (long long)t.y * 64 + threadIdx.x, 2 * a.nx + 2 * (a.ny - 2), band_nodes(a));
+    side_block(a, t.y, band_nodes(a));
 }
 
 // Wall nodes of every run from a table of (run, block of 64 boundary nodes), for walls of one
@@ -1201,9 +1270,10 @@ template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipS
     const int nring = 2 * a.nx + 2 * (a.ny - 2);
     long long nband = 0;
     for (int q = 0; q < a.nrect; q++) nband += (long long)a.rect[q][2] * a.rect[q][3];
-    const long long n = nring + nband + (stencil && a.has_boundary ? a.b.nU : 0);
+    const int nct = ring_column_tiles(a.nx, a.ny, a.pops != nullptr);
+    const long long n = (nct ? 2 * a.nx : nring) + nband + (stencil && a.has_boundary ? a.b.nU : 0);
     // single-wave workgroups: they fit beside a running bulk launch (see wall_nodes_kernel)
-    hipLaunchKernelGGL((side_kernel<S, T>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, a, nring, (int)nband);
+    hipLaunchKernelGGL((side_kernel<S, T>), dim3((unsigned)(nct + (n + 63) / 64)), dim3(64), 0, st, a, (int)nband);
     return hipGetLastError();
 }
 template <class S, class T> hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st)
