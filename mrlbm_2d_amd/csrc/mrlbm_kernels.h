@@ -58,7 +58,7 @@ template <class S, class T> struct StepArgs {
     double inlet_prev_c;       // its mean counter
     double *rho_inf_mean;
     double *facc;              // batched runs: running sums of Fd, Fl, Fl^2 and the step count
-    int nrect;                 // band kernel: node rectangles x0, y0, w, h
+    int nrect;                 // side kernel: band rectangles x0, y0, w, h
     int rect[6][4];
     const T *chi;              // Brinkman solid fraction (per-node bulk kernel only), or null
     BoundaryDev b;
