@@ -66,6 +66,7 @@ template <class S, class T> struct AdjArgs {
     T *Ufull, *outm, *lub_own, *lub, *lMi, *lout;
     const int32_t *rev_dep_start, *rev_dep, *rev_u_start, *rev_u;
     T wJ;                // objective weight of this step's drag (0 outside the window)
+    int stencil;         // adj_prep_kernel also fills the forward's stencil table (b.U)
 };
 
 // Conserved-coordinate weights v_k of the reconstruction: d f_k / d(rho, jx, jy, Pxx, Pyy, Pxy) / w_k.
@@ -133,6 +134,7 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_prep_kern
         T f[Q], m[6];
         gather_interior(a, b.uij[2 * t], b.uij[2 * t + 1], f);
         F::moments(f, m);
+        if (d.stencil) store_stencil_moments(a, t, m); // what stencil_moments_kernel writes
         m[0] = F::rho_of(m[0]);
 #pragma unroll
         for (int p = 0; p < 6; p++) d.Ufull[p * b.nU + t] = m[p];
@@ -457,11 +459,17 @@ template <class S, class T>
 static int adjoint_step(const StepArgs<S, T> &a, const AdjArgs<S, T> &d, bool flat, hipStream_t st)
 {
     const BoundaryDev &b = a.b;
-    hipError_t e = launch_stencil_moments<S, T>(a, st);
-    if (e != hipSuccess) return MRLBM_E_HIP;
-    if (b.nlevels > 1) hipLaunchKernelGGL((adj_wallvel_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
+    // one dependency level: the stencil table comes with adj_prep_kernel (same gathers), one
+    // launch fewer per reverse step; otherwise the wall velocities need it first
+    AdjArgs<S, T> dp = d;
+    dp.stencil = b.nlevels <= 1 ? 1 : 0;
+    if (!dp.stencil) {
+        hipError_t e = launch_stencil_moments<S, T>(a, st);
+        if (e != hipSuccess) return MRLBM_E_HIP;
+        hipLaunchKernelGGL((adj_wallvel_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
+    }
     const int nprep = std::max(std::max(d.ns * Q, b.nU), a.ny);
-    hipLaunchKernelGGL((adj_prep_kernel<S, T>), dim3((nprep + 255) / 256), dim3(256), 0, st, a, d);
+    hipLaunchKernelGGL((adj_prep_kernel<S, T>), dim3((nprep + 255) / 256), dim3(256), 0, st, a, dp);
     const int nloc = (b.nbct + d.nedge) * 13;
     hipLaunchKernelGGL((adj_local_kernel<S, T>), dim3((nloc + 255) / 256), dim3(256), 0, st, a, d);
     if (flat)

@@ -140,6 +140,23 @@ template <class S, class T> __device__ void gather_interior(const StepArgs<S, T>
     }
 }
 
+// Pre-collision rho, u and rotated Mxx', Myy' of stencil node u (moments m as F::moments
+// gives them) into the wall's stencil table (2d_mrlbm.f90:840-869 for the rotation).
+template <class S, class T> __device__ __forceinline__ void store_stencil_moments(const StepArgs<S, T> &a, int u, const T m[6])
+{
+    const BoundaryDev &b = a.b;
+    T *U = (T *)b.U;
+    const int nU = b.nU;
+    U[u] = m[0];
+    U[nU + u] = m[1];
+    U[2 * nU + u] = m[2];
+    if (b.rotated && b.mom_interp) {
+        const T c2 = (T)b.urot[3 * u], s2 = (T)b.urot[3 * u + 1], sc = (T)b.urot[3 * u + 2];
+        U[3 * nU + u] = ((m[3] * c2) + (m[4] * s2)) + (m[5] * sc);
+        U[4 * nU + u] = ((m[3] * s2) + (m[4] * c2)) - (m[5] * sc);
+    }
+}
+
 // Dirichlet values and the regularized edge BC, in the reference's order
 // outlet -> inlet -> top -> bottom with the last writer winning (2d_mrlbm.f90:247-351).
 template <class S, class T>

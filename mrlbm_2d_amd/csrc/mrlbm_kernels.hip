@@ -662,19 +662,10 @@ template <class S, class T> __device__ void stencil_moments_body(const StepArgs<
     using F = typename FormOf<T>::F;
     const BoundaryDev &b = a.b;
     if (u >= b.nU) return;
-    T *U = (T *)b.U;
-    const int nU = b.nU;
     T f[Q], m[6];
     gather_interior(a, b.uij[2 * u], b.uij[2 * u + 1], f);
     F::moments(f, m);
-    U[u] = m[0];
-    U[nU + u] = m[1];
-    U[2 * nU + u] = m[2];
-    if (b.rotated && b.mom_interp) { // rotated moments at the stencil node (:840-869)
-        const T c2 = (T)b.urot[3 * u], s2 = (T)b.urot[3 * u + 1], sc = (T)b.urot[3 * u + 2];
-        U[3 * nU + u] = ((m[3] * c2) + (m[4] * s2)) + (m[5] * sc);
-        U[4 * nU + u] = ((m[3] * s2) + (m[4] * c2)) - (m[5] * sc);
-    }
+    store_stencil_moments(a, u, m);
 }
 
 template <class S, class T> __global__ void __launch_bounds__(256) stencil_moments_kernel(StepArgs<S, T> a)
