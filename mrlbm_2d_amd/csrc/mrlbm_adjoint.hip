@@ -43,6 +43,7 @@ struct AdjointWork {
     int ns = 0;
     bool flat = false;         // one dependency level and no stencil node in column nx-2
     void *E = nullptr;         // ns * 9
+    void *E2 = nullptr;        // the same for the other step parity (a step's prep runs beside the previous bulk)
     int nedge = 0;
     int32_t *edge_ij = nullptr;
     void *Ufull = nullptr;     // 6 * nU pre-collision moments of the stencil nodes
@@ -124,10 +125,9 @@ __device__ __forceinline__ void load_lam(const StepArgs<S, T> &a, const T *lam, 
 // ---------------------------------------------------------------------------------
 // Pre-collision moments of the stencil nodes and of column nx-2 at the step's start
 // state; clears the special rows.
-template <class S, class T> __global__ void __launch_bounds__(256) adj_prep_kernel(StepArgs<S, T> a, AdjArgs<S, T> d)
+template <class S, class T> __device__ void adj_prep_body(const StepArgs<S, T> &a, const AdjArgs<S, T> &d, int t)
 {
     using F = typename FormOf<T>::F;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < d.ns * Q) d.E[t] = (T)0.0;
     const BoundaryDev &b = a.b;
     if (t < b.nU) {
@@ -147,6 +147,11 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_prep_kern
         d.outm[3 * t + 1] = m[1];
         d.outm[3 * t + 2] = m[2];
     }
+}
+
+template <class S, class T> __global__ void __launch_bounds__(256) adj_prep_kernel(StepArgs<S, T> a, AdjArgs<S, T> d)
+{
+    adj_prep_body(a, d, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 template <class S, class T> __global__ void __launch_bounds__(256) adj_wallvel_kernel(StepArgs<S, T> a)
@@ -403,10 +408,9 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_interp_fl
 }
 
 // Reverse step of every node: gather of the downstream population adjoints, plus dJ/dchi.
-template <class S, class T> __global__ void __launch_bounds__(256) adj_bulk_kernel(StepArgs<S, T> a, AdjArgs<S, T> d)
+template <class S, class T> __device__ void adj_bulk_body(const StepArgs<S, T> &a, const AdjArgs<S, T> &d, long long t)
 {
     using F = typename FormOf<T>::F;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)a.nx * a.ny) return;
     const int i = (int)(t % a.nx), j = (int)(t / a.nx);
     const long long ns_ = (long long)j * a.pitch + i; // this node
@@ -455,21 +459,42 @@ template <class S, class T> __global__ void __launch_bounds__(256) adj_bulk_kern
     if (a.chi && bulk[0]) d.G[ns_] += (double)lchi0;
 }
 
+template <class S, class T> __global__ void __launch_bounds__(256) adj_bulk_kernel(StepArgs<S, T> a, AdjArgs<S, T> d)
+{
+    adj_bulk_body(a, d, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The bulk gather of one reverse step and, in the workgroups past nbulk, the prep of the
+// next one (an, dn: the state before; its own special-row buffer): nothing the gather reads
+// is written by that prep.
 template <class S, class T>
-static int adjoint_step(const StepArgs<S, T> &a, const AdjArgs<S, T> &d, bool flat, hipStream_t st)
+__global__ void __launch_bounds__(256) adj_bulk_prep_kernel(StepArgs<S, T> a, AdjArgs<S, T> d, StepArgs<S, T> an,
+                                                           AdjArgs<S, T> dn, int nbulk)
+{
+    if ((int)blockIdx.x < nbulk) adj_bulk_body(a, d, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+    else adj_prep_body(an, dn, ((int)blockIdx.x - nbulk) * (int)blockDim.x + (int)threadIdx.x);
+}
+
+template <class S, class T>
+static int adjoint_step(const StepArgs<S, T> &a, const AdjArgs<S, T> &d, bool flat, hipStream_t st,
+                        const StepArgs<S, T> *an = nullptr, const AdjArgs<S, T> *dn = nullptr, bool prepped = false)
 {
     const BoundaryDev &b = a.b;
-    // one dependency level: the stencil table comes with adj_prep_kernel (same gathers), one
-    // launch fewer per reverse step; otherwise the wall velocities need it first
-    AdjArgs<S, T> dp = d;
-    dp.stencil = b.nlevels <= 1 ? 1 : 0;
-    if (!dp.stencil) {
-        hipError_t e = launch_stencil_moments<S, T>(a, st);
-        if (e != hipSuccess) return MRLBM_E_HIP;
-        hipLaunchKernelGGL((adj_wallvel_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
-    }
+    // one dependency level: the stencil table comes with the prep (same gathers), and the
+    // prep of the next step (an, dn) with this step's bulk gather; otherwise the wall
+    // velocities need the stencil table first, in launches of their own
+    const bool one = b.nlevels <= 1;
     const int nprep = std::max(std::max(d.ns * Q, b.nU), a.ny);
-    hipLaunchKernelGGL((adj_prep_kernel<S, T>), dim3((nprep + 255) / 256), dim3(256), 0, st, a, dp);
+    if (!prepped) {
+        AdjArgs<S, T> dp = d;
+        dp.stencil = one ? 1 : 0;
+        if (!one) {
+            hipError_t e = launch_stencil_moments<S, T>(a, st);
+            if (e != hipSuccess) return MRLBM_E_HIP;
+            hipLaunchKernelGGL((adj_wallvel_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
+        }
+        hipLaunchKernelGGL((adj_prep_kernel<S, T>), dim3((nprep + 255) / 256), dim3(256), 0, st, a, dp);
+    }
     const int nloc = (b.nbct + d.nedge) * 13;
     hipLaunchKernelGGL((adj_local_kernel<S, T>), dim3((nloc + 255) / 256), dim3(256), 0, st, a, d);
     if (flat)
@@ -477,7 +502,16 @@ static int adjoint_step(const StepArgs<S, T> &a, const AdjArgs<S, T> &d, bool fl
     else
         hipLaunchKernelGGL((adj_interp_kernel<S, T>), dim3(1), dim3(256), 0, st, a, d);
     const long long N = (long long)a.nx * a.ny;
-    hipLaunchKernelGGL((adj_bulk_kernel<S, T>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, a, d);
+    const int nbulk = (int)((N + 255) / 256);
+    if (an && one) {
+        AdjArgs<S, T> dp = *dn;
+        dp.stencil = 1;
+        const int nnext = std::max(std::max(dn->ns * Q, an->b.nU), an->ny);
+        hipLaunchKernelGGL((adj_bulk_prep_kernel<S, T>), dim3((unsigned)(nbulk + (nnext + 255) / 256)), dim3(256), 0, st,
+                           a, d, *an, dp, nbulk);
+    } else {
+        hipLaunchKernelGGL((adj_bulk_kernel<S, T>), dim3((unsigned)nbulk), dim3(256), 0, st, a, d);
+    }
     return hipGetLastError() == hipSuccess ? 0 : MRLBM_E_HIP;
 }
 
@@ -597,6 +631,7 @@ template <class S, class T> static int adjoint_setup(mrlbm_handle *h, int K, int
     rc |= upload_vec(h, w, ul, &w->rev_u);
     if (rc) return rc;
     w->E = walloc<T>(w, (size_t)ns * Q, e);
+    if (e == hipSuccess) w->E2 = walloc<T>(w, (size_t)ns * Q, e);
     HIPCHK(e);
     w->Ufull = walloc<T>(w, (size_t)6 * std::max(bt.nU, 1), e);
     HIPCHK(e);
@@ -712,20 +747,22 @@ static int adjoint_typed(mrlbm_handle *h, const mrlbm_adjoint_opts *o, double *d
             const int rc = recompute(s, h->st);
             if (rc) return rc;
         }
-        for (int t = len - 1; t >= 0; t--) {
-            const int step = s * K + t; // state step -> step + 1
-            auto a = base_args<S, T>(h);
+        // arguments of the reverse of step s*K + t (state t of the segment -> t + 1), with the
+        // lam and special-row buffers of parity lcv
+        auto make = [&](int t, int lcv, StepArgs<S, T> &a, AdjArgs<S, T> &d) {
+            const int step = s * K + t;
+            a = base_args<S, T>(h);
             a.src = seg + st_words * t;
             a.dst = nullptr;
             a.pops = nullptr;
-            AdjArgs<S, T> d{};
+            d = AdjArgs<S, T>{};
             d.m1 = seg + st_words * (t + 1);
-            d.lam1 = (const T *)w->lam[lc];
-            d.lam0 = (T *)w->lam[lc ^ 1];
+            d.lam1 = (const T *)w->lam[lcv];
+            d.lam0 = (T *)w->lam[lcv ^ 1];
             d.G = w->G;
             d.guin = w->guin;
             d.spec = w->spec;
-            d.E = (T *)w->E;
+            d.E = (T *)(lcv ? w->E2 : w->E);
             d.ns = w->ns;
             d.nedge = w->nedge;
             d.edge_ij = w->edge_ij;
@@ -740,7 +777,22 @@ static int adjoint_typed(mrlbm_handle *h, const mrlbm_adjoint_opts *o, double *d
             d.rev_u_start = w->rev_u_start;
             d.rev_u = w->rev_u;
             d.wJ = step >= nsteps - window ? wJ : (T)0.0;
-            const int rc = adjoint_step<S, T>(a, d, w->flat, h->st);
+        };
+        // within the segment, each step's bulk gather also does the next step's prep
+        bool prepped = false;
+        for (int t = len - 1; t >= 0; t--) {
+            StepArgs<S, T> a, an;
+            AdjArgs<S, T> d, dn;
+            make(t, lc, a, d);
+            int rc;
+            if (t >= 1) {
+                make(t - 1, lc ^ 1, an, dn);
+                rc = adjoint_step<S, T>(a, d, w->flat, h->st, &an, &dn, prepped);
+                prepped = h->bt.nlevels <= 1;
+            } else {
+                rc = adjoint_step<S, T>(a, d, w->flat, h->st, nullptr, nullptr, prepped);
+                prepped = false;
+            }
             if (rc) FAIL(rc, "adjoint kernel launch failed");
             lc ^= 1;
         }
