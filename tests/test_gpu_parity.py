@@ -301,3 +301,35 @@ def test_u_statistics_in_two_step_pairs_match_single_steps(oracle_mod, precision
         assert ns == n
         np.testing.assert_allclose(mean_ux, su / n, rtol=1e-13, atol=1e-17)
         np.testing.assert_allclose(mean_uy, sv / n, rtol=1e-13, atol=1e-17)
+
+
+def test_full_size_recording_in_pairs_matches_single_steps():
+    """8192 x 2048 FP64 (the benchmark channel, D = 128) with everything recorded: forces,
+    surface pressure, four probes and u statistics every 3rd step inside a window that
+    opens and closes mid-pair.  Two steps per pass against one step per pass: fields,
+    force, probe and surface-pressure series and the statistics bit for bit."""
+    import bench
+    out = {}
+    for tb in (0, -1):
+        print(f"[gpu] 8192x2048 FP64 recording, time_blocking={tb}", flush=True)
+        lat = bench.make_lattice(capi.MRLBM_FP64, time_blocking=tb)
+        try:
+            o = capi.default_step_opts()
+            o.post_process, o.statsbegin, o.statsend = 1, 41, 220
+            o.record_surface_p, o.stats_interval = 1, 3
+            o.nprobe = 4
+            for k, (i, j) in enumerate(((1160, 1026), (1300, 1026), (2000, 3), (8192, 700))):
+                o.probe_i[k], o.probe_j[k] = i, j
+            lat.step(2, 240, o)
+            out[tb] = (lat.fields(), lat.forces(), lat.probes(), lat.surface_p_series(), lat.stats())
+        finally:
+            lat.close()
+    (f0, fo0, p0, s0, st0), (f1, fo1, p1, s1, st1) = out[0], out[-1]
+    for a, b in zip(f1, f0):
+        np.testing.assert_array_equal(a, b)
+    for group1, group0 in ((fo1, fo0), (p1, p0), (s1, s0)):
+        for a, b in zip(group1, group0):
+            np.testing.assert_array_equal(a, b)
+    assert len(fo1[0]) == 180 and st1[4] == st0[4] == 60
+    for a, b in zip(st1[:4], st0[:4]):
+        np.testing.assert_array_equal(a, b)
