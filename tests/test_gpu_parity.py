@@ -333,3 +333,24 @@ def test_full_size_recording_in_pairs_matches_single_steps():
     assert len(fo1[0]) == 180 and st1[4] == st0[4] == 60
     for a, b in zip(st1[:4], st0[:4]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("nx,ny,xper,yper", [(6, 5, False, False), (12, 10, False, False), (12, 10, True, False),
+                                             (6, 9, False, True), (17, 16, False, False)])
+def test_small_channels_match_oracle(oracle_mod, nx, ny, xper, yper):
+    """Plain channels below the sizes of the two-step pass (16 x 16) and of the ring-column
+    tiles (8 wide), plus the smallest two-step grid: FP64 fields bit for bit against the
+    oracle after 60 steps, with x or y periodic wraps."""
+    p = dict(gc.load("chan")["params"])
+    p.update(nx=nx, ny=ny, x_periodic=xper, y_periodic=yper)
+    o = oracle_mod.Oracle(gc.oracle_params(oracle_mod, p))
+    assert (o.nx, o.ny) == (nx, ny)
+    o.run(2, 60)
+    orho, oux, ouy, *_ = o.moments()
+    with Lattice(params_from_namelist(p)) as lat:
+        print(f"[gpu] small channel {nx}x{ny} xper={xper} yper={yper}", flush=True)
+        lat.step(2, 60, step_opts_from_namelist(p))
+        rho, ux, uy = lat.fields()
+    np.testing.assert_array_equal(rho, orho)
+    np.testing.assert_array_equal(ux, oux)
+    np.testing.assert_array_equal(uy, ouy)
