@@ -196,7 +196,7 @@ static void free_all(mrlbm_handle *h)
 {
     adjoint_free(h);
     for (void *p : {h->S[0], h->S[1], h->pops, h->ghost, h->uin, (void *)h->d_boxbits, (void *)h->d_rec, (void *)h->d_inlet[0],
-                    (void *)h->d_inlet[1], (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red, h->chi, h->S2})
+                    (void *)h->d_inlet[1], (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red, h->chi, h->S2, (void *)h->d_dense})
         if (p) (void)hipFree(p);
     for (void *p : h->allocs)
         if (p) (void)hipFree(p);
@@ -447,23 +447,29 @@ extern "C" int mrlbm_get_node_class(const mrlbm_handle *h, int8_t *cls)
 }
 
 // dense (nx, ny) host <-> pitched device plane
+// (the layout change runs on the device: one contiguous host transfer per plane)
+static double *dense_staging(mrlbm_handle *h)
+{
+    if (!h->d_dense && hipMalloc(&h->d_dense, (size_t)h->nx * h->ny * sizeof(double)) != hipSuccess) h->d_dense = nullptr;
+    return h->d_dense;
+}
 template <class S> static int put_plane(mrlbm_handle *h, S *dplane, const double *src, double shift)
 {
-    std::vector<S> tmp((size_t)h->pitch * h->ny, (S)0);
-    for (int j = 0; j < h->ny; j++)
-        for (int i = 0; i < h->nx; i++)
-            tmp[(size_t)j * h->pitch + i] = (S)(src[(size_t)j * h->nx + i] - shift);
-    HIPCHK(hipMemcpy(dplane, tmp.data(), tmp.size() * sizeof(S), hipMemcpyHostToDevice));
+    double *d = dense_staging(h);
+    if (!d) FAIL(MRLBM_E_NOMEM, "no device memory for the host-copy staging plane");
+    HIPCHK(hipMemcpyAsync(d, src, (size_t)h->nx * h->ny * sizeof(double), hipMemcpyHostToDevice, h->st));
+    HIPCHK(launch_plane_from_dense<S>(d, dplane, h->nx, h->ny, h->pitch, shift, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
-template <class S> static int get_plane(const mrlbm_handle *h, const S *dplane, double *dst, double shift)
+template <class S> static int get_plane(const mrlbm_handle *hc, const S *dplane, double *dst, double shift)
 {
-    std::vector<S> tmp((size_t)h->pitch * h->ny);
-    if (hipMemcpy(tmp.data(), dplane, tmp.size() * sizeof(S), hipMemcpyDeviceToHost) != hipSuccess)
-        return MRLBM_E_HIP;
-    for (int j = 0; j < h->ny; j++)
-        for (int i = 0; i < h->nx; i++)
-            dst[(size_t)j * h->nx + i] = shift + (double)tmp[(size_t)j * h->pitch + i];
+    auto *h = const_cast<mrlbm_handle *>(hc);
+    double *d = dense_staging(h);
+    if (!d) FAIL(MRLBM_E_NOMEM, "no device memory for the host-copy staging plane");
+    HIPCHK(launch_dense_from_plane<S>(dplane, d, h->nx, h->ny, h->pitch, shift, h->st));
+    HIPCHK(hipMemcpyAsync(dst, d, (size_t)h->nx * h->ny * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
 

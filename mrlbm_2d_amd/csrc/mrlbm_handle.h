@@ -81,6 +81,7 @@ struct mrlbm_handle {
     void *chi = nullptr;      // Brinkman solid fraction per node (step type), or null
     void *S2 = nullptr;       // scratch state of the two-step pass (state t+1 on the bands)
     int tb_H = 0, nstrips2 = 0, ncg2 = 0, ncb2 = 0; // two-step pass: strip height, strips, 64-column waves, blocks per strip
+    double *d_dense = nullptr;              // nx*ny FP64 staging for the host copies (on first use)
     int rect1[5][4] = {}, rect2[5][4] = {}; // band rectangles of sub-steps 1 and 2
     int nrect1 = 0, nrect2 = 0;
     AdjointWork *adj = nullptr; // buffers of the discrete adjoint, allocated on first use

@@ -74,6 +74,11 @@ template <class S, class T>
 hipError_t launch_edges(const StepArgs<S, T> &a, hipStream_t st);
 template <class S, class T>
 hipError_t launch_inlet_fold(const StepArgs<S, T> &a, hipStream_t st);
+// Pitched state plane <-> dense (nx, ny) FP64 array (with a shift), on the device.
+template <class S>
+hipError_t launch_dense_from_plane(const S *src, double *dst, int nx, int ny, int pitch, double shift, hipStream_t st);
+template <class S>
+hipError_t launch_plane_from_dense(const double *src, S *dst, int nx, int ny, int pitch, double shift, hipStream_t st);
 // The surface-pressure smoothing weights and their sums, once per handle (they depend on
 // the wall angles only).
 hipError_t launch_smooth_weights(const BoundaryDev &b, double *w, double *sw, hipStream_t st);

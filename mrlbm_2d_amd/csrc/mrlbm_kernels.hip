@@ -1069,6 +1069,26 @@ template <class S, class T> __global__ void __launch_bounds__(256) pair_samples_
     }
 }
 
+// A pitched state plane to a dense (nx, ny) FP64 array (i fastest) and back, shifted by
+// `shift` (the FP32 form stores rho - 1): the host copies of the C-ABI are then one
+// contiguous transfer.  Padding columns of the pitched plane are written as zeros.
+template <class S>
+__global__ void __launch_bounds__(256) dense_from_plane_kernel(const S *src, double *dst, int nx, int ny, int pitch, double shift)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nx * ny) return;
+    const int i = (int)(t % nx), j = (int)(t / nx);
+    dst[t] = shift + (double)src[(long long)j * pitch + i];
+}
+template <class S>
+__global__ void __launch_bounds__(256) plane_from_dense_kernel(const double *src, S *dst, int nx, int ny, int pitch, double shift)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)pitch * ny) return;
+    const int i = (int)(t % pitch), j = (int)(t / pitch);
+    dst[t] = i < nx ? (S)(src[(long long)j * nx + i] - shift) : (S)0;
+}
+
 template <class S, class T> __global__ void __launch_bounds__(256) inlet_fold_kernel(StepArgs<S, T> a)
 {
     __shared__ double sh[16];
@@ -1309,6 +1329,27 @@ hipError_t launch_pair_probes(const StepArgs<S, T> &a, const S *s2, int slot2, h
     hipLaunchKernelGGL((pair_probes_kernel<S, T>), dim3(1), dim3(64), 0, st, a, s2, slot2);
     return hipGetLastError();
 }
+template <class S>
+hipError_t launch_dense_from_plane(const S *src, double *dst, int nx, int ny, int pitch, double shift, hipStream_t st)
+{
+    const long long n = (long long)nx * ny;
+    hipLaunchKernelGGL((dense_from_plane_kernel<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, nx, ny,
+                       pitch, shift);
+    return hipGetLastError();
+}
+template <class S>
+hipError_t launch_plane_from_dense(const double *src, S *dst, int nx, int ny, int pitch, double shift, hipStream_t st)
+{
+    const long long n = (long long)pitch * ny;
+    hipLaunchKernelGGL((plane_from_dense_kernel<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, nx, ny,
+                       pitch, shift);
+    return hipGetLastError();
+}
+template hipError_t launch_dense_from_plane<float>(const float *, double *, int, int, int, double, hipStream_t);
+template hipError_t launch_dense_from_plane<double>(const double *, double *, int, int, int, double, hipStream_t);
+template hipError_t launch_plane_from_dense<float>(const double *, float *, int, int, int, double, hipStream_t);
+template hipError_t launch_plane_from_dense<double>(const double *, double *, int, int, int, double, hipStream_t);
+
 template <class S, class T> hipError_t launch_inlet_fold(const StepArgs<S, T> &a, hipStream_t st)
 {
     hipLaunchKernelGGL((inlet_fold_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
