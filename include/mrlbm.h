@@ -134,6 +134,14 @@ int mrlbm_get_moments(const mrlbm_handle *h, double *rho, double *ux, double *uy
                       double *mxx, double *myy, double *mxy);
 /* Post-stream populations f(nx,ny,0:8) for restart.dat (write_restart_file, :2186-2202). */
 int mrlbm_get_populations(const mrlbm_handle *h, double *f);
+/* The same populations, written to the same f, without allocating anything (no counterpart in
+ * the reference; for callers that cannot allocate device memory: code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  mrlbm_get_populations allocates and frees
+ * 72 B per node on every call; this variant stages bands of rows through a fixed 8 MiB array
+ * per GPU that the library holds until the process ends.  Limit: nx <= 116508 (one row of nine
+ * FP64 populations must fit the array), else MRLBM_E_ARG.  Every call on one GPU shares that
+ * GPU's array: calls that overlap in time on one GPU (from several host threads) race. */
+int mrlbm_get_populations_noalloc(mrlbm_handle *h, double *f);
 
 /* Force series recorded since the last call (evaluate_forces + sums, :481-494):
  * up to max entries of (iter, Fd, Fl); *count gets the number returned. */

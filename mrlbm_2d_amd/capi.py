@@ -24,7 +24,8 @@ EXPORTS = (
     "mrlbm_create", "mrlbm_destroy", "mrlbm_last_error", "mrlbm_abi_version", "mrlbm_default_params",
     "mrlbm_default_step_opts", "mrlbm_get_info", "mrlbm_get_boundary", "mrlbm_get_node_class",
     "mrlbm_set_populations", "mrlbm_set_moments", "mrlbm_step", "mrlbm_synchronize",
-    "mrlbm_get_fields", "mrlbm_get_moments", "mrlbm_get_populations", "mrlbm_get_forces",
+    "mrlbm_get_fields", "mrlbm_get_moments", "mrlbm_get_populations", "mrlbm_get_populations_noalloc",
+    "mrlbm_get_forces",
     "mrlbm_get_surface_p", "mrlbm_get_surface_p_series", "mrlbm_get_rho_inf_mean",
     "mrlbm_get_probes", "mrlbm_get_stats", "mrlbm_reset_stats", "mrlbm_max_ux",
     "mrlbm_check_finite", "mrlbm_set_timing", "mrlbm_last_step_ms", "mrlbm_last_bulk_ms",
@@ -137,6 +138,7 @@ def load(path: str = LIB_PATH):
     L.mrlbm_get_fields.argtypes = [vp, dp, dp, dp]
     L.mrlbm_get_moments.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.mrlbm_get_populations.argtypes = [vp, dp]
+    L.mrlbm_get_populations_noalloc.argtypes = [vp, dp]
     L.mrlbm_get_forces.argtypes = [vp, ip, dp, dp, C.c_int32, ip]
     L.mrlbm_get_surface_p.argtypes = [vp, ip, dp, dp]
     L.mrlbm_get_surface_p_series.argtypes = [vp, ip, dp, dp, C.c_int32, ip]
@@ -306,6 +308,12 @@ class Lattice:
     def populations(self):
         a = np.zeros(9 * self.N)
         self._check(self._L.mrlbm_get_populations(self._h, _dp(a)))
+        return a.reshape(self.nx, self.ny, 9, order="F")
+
+    def populations_noalloc(self):
+        """As populations(), through the entry point that allocates nothing on the device."""
+        a = np.zeros(9 * self.N)
+        self._check(self._L.mrlbm_get_populations_noalloc(self._h, _dp(a)))
         return a.reshape(self.nx, self.ny, 9, order="F")
 
     # -- records

@@ -7,11 +7,39 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 namespace {
 thread_local std::string g_create_error;
+} // namespace
+
+// Staging array of mrlbm_get_populations_noalloc: one per GPU, in the library's code object,
+// held until the process ends.  Host code never uses its name as an address (in host code
+// the name is a stand-in, not device memory): each device's address comes from
+// hipGetSymbolAddress, taken once per device at mrlbm_create with that device current.
+constexpr size_t NOALLOC_SCRATCH_WORDS = (size_t)1 << 20; // 8 MiB of FP64
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_WORDS];
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on `dev`, which must be the current device.
+hipError_t scratch_for_device(int dev, double **out)
+{
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
 } // namespace
 
 extern "C" int mrlbm_abi_version(void) { return MRLBM_ABI_VERSION; }
@@ -357,6 +385,13 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
     else h->nu = uref * (double)(p->ny - 1) / p->Re;
     h->omega = p->omega > 0 ? p->omega : 1.0 / ((3.0 * h->nu) + (1.0 / 2.0));
     if (p->omega > 0) h->nu = (1.0 / p->omega - 0.5) / 3.0;
+    if (hipGetDevice(&h->device) != hipSuccess) return fail(MRLBM_E_HIP, "hipGetDevice failed");
+    {
+        double *sp = nullptr;
+        const hipError_t e = scratch_for_device(h->device, &sp);
+        if (e != hipSuccess)
+            return fail(MRLBM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
     if (hipStreamCreate(&h->st) != hipSuccess || hipStreamCreate(&h->st2) != hipSuccess)
         return fail(MRLBM_E_HIP, "hipStreamCreate failed");
     // the stream-ordering events order work on this device only: a device-scope release
@@ -851,7 +886,7 @@ template <class S, class T> static int get_pops_typed(mrlbm_handle *h, double *f
     double *d = nullptr;
     HIPCHK(hipMalloc(&d, (size_t)Q * N * sizeof(double)));
     auto a = base_args<S, T>(h);
-    hipError_t e = launch_get_populations<S, T>(a, d, h->st);
+    hipError_t e = launch_get_populations<S, T>(a, d, 0, h->ny, h->st);
     if (e == hipSuccess) e = hipStreamSynchronize(h->st);
     if (e == hipSuccess) e = hipMemcpy(f, d, (size_t)Q * N * sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(d);
@@ -864,6 +899,43 @@ extern "C" int mrlbm_get_populations(const mrlbm_handle *hc, double *f)
     auto *h = const_cast<mrlbm_handle *>(hc);
     if (!h || !f) return MRLBM_E_ARG;
     return h->f32 ? get_pops_typed<float, float>(h, f) : get_pops_typed<double, double>(h, f);
+}
+
+// Same populations as mrlbm_get_populations, staged through this GPU's `scratch` array in
+// bands of whole rows: per band one kernel writes f(nx, rows, 0:8) into the array, then nine
+// contiguous copies (one per k) land in f.  Every staged word is written before it is read,
+// so nothing needs clearing between calls.  No device or host allocation.
+template <class S, class T> static int get_pops_noalloc_typed(mrlbm_handle *h, double *f)
+{
+    const size_t row_words = (size_t)Q * h->nx;
+    if (row_words > NOALLOC_SCRATCH_WORDS)
+        FAIL(MRLBM_E_ARG, "nx too large for mrlbm_get_populations_noalloc: one row of 9 populations must fit "
+                          "its 8 MiB staging array (nx <= 116508)");
+    double *sp = nullptr;
+    {
+        const hipError_t e = scratch_for_device(h->device, &sp);
+        if (e != hipSuccess) FAIL(MRLBM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    const int band = (int)std::min<size_t>(NOALLOC_SCRATCH_WORDS / row_words, (size_t)h->ny);
+    const size_t N = (size_t)h->nx * h->ny;
+    auto a = base_args<S, T>(h);
+    for (int j0 = 0; j0 < h->ny; j0 += band) {
+        const int rows = std::min(band, h->ny - j0);
+        const size_t bn = (size_t)h->nx * rows;
+        HIPCHK(launch_get_populations<S, T>(a, sp, j0, rows, h->st));
+        for (int k = 0; k < Q; k++)
+            HIPCHK(hipMemcpyAsync(f + k * N + (size_t)j0 * h->nx, sp + k * bn, bn * sizeof(double),
+                                  hipMemcpyDeviceToHost, h->st));
+        // the next band's kernel overwrites the array: its copies must have landed
+        HIPCHK(hipStreamSynchronize(h->st));
+    }
+    return 0;
+}
+
+extern "C" int mrlbm_get_populations_noalloc(mrlbm_handle *h, double *f)
+{
+    if (!h || !f) return MRLBM_E_ARG;
+    return h->f32 ? get_pops_noalloc_typed<float, float>(h, f) : get_pops_noalloc_typed<double, double>(h, f);
 }
 
 extern "C" int mrlbm_get_forces(mrlbm_handle *h, int32_t *iters, double *fd, double *fl, int32_t max,

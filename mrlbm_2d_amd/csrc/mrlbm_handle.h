@@ -35,6 +35,7 @@ struct mrlbm_handle {
     BoundaryTables bt;
     bool has_b = false;
     double nu = 0, omega = 0;
+    int device = 0;             // HIP device current at mrlbm_create (the handle's memory lives there)
     hipStream_t st = nullptr;   // bulk kernel and every host-facing operation
     hipStream_t st2 = nullptr;  // side work (ring, bands, stencil, wall), concurrent with the bulk
     hipEvent_t evA = nullptr, evB = nullptr; // end of the last bulk / side work

@@ -90,8 +90,9 @@ hipError_t launch_pair_probes(const StepArgs<S, T> &a, const S *s2, int slot2, h
 // rectangles that the two-step bulk writes (a: sub-step 1; s2: state t+2); after both.
 template <class S, class T>
 hipError_t launch_pair_samples(const StepArgs<S, T> &a, const S *s2, hipStream_t st);
+// Post-stream populations of rows j0 .. j0+rows-1 into a dense band f(nx, rows, 0:8).
 template <class S, class T>
-hipError_t launch_get_populations(const StepArgs<S, T> &a, double *f_dense, hipStream_t st);
+hipError_t launch_get_populations(const StepArgs<S, T> &a, double *f_band, int j0, int rows, hipStream_t st);
 template <class S, class T>
 hipError_t launch_ghost_from_state(const StepArgs<S, T> &a, T *ghost, hipStream_t st);
 template <class S, class T>

@@ -1101,14 +1101,16 @@ template <class S, class T> __global__ void __launch_bounds__(256) inlet_fold_ke
     }
 }
 
+// Post-stream populations of rows j0 .. j0+rows-1 as a dense band f(nx, rows, 0:8) (x fastest,
+// then the band's rows, then k); the whole grid when j0 = 0 and rows = ny.
 template <class S, class T>
-__global__ void __launch_bounds__(256) get_populations_kernel(StepArgs<S, T> a, double *f)
+__global__ void __launch_bounds__(256) get_populations_kernel(StepArgs<S, T> a, double *f, int j0, int rows)
 {
     using F = typename FormOf<T>::F;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long N = (long long)a.nx * a.ny;
+    const long long N = (long long)a.nx * rows;
     if (t >= N) return;
-    const int i = (int)(t % a.nx), j = (int)(t / a.nx);
+    const int i = (int)(t % a.nx), j = j0 + (int)(t / a.nx);
     for (int k = 0; k < Q; k++) {
         const T v = pull_rule(a, i, j, k);
         f[k * N + t] = F::dev ? (double)F::base(k) + (double)v : (double)v;
@@ -1355,10 +1357,13 @@ template <class S, class T> hipError_t launch_inlet_fold(const StepArgs<S, T> &a
     hipLaunchKernelGGL((inlet_fold_kernel<S, T>), dim3(1), dim3(256), 0, st, a);
     return hipGetLastError();
 }
-template <class S, class T> hipError_t launch_get_populations(const StepArgs<S, T> &a, double *f, hipStream_t st)
+template <class S, class T>
+hipError_t launch_get_populations(const StepArgs<S, T> &a, double *f, int j0, int rows, hipStream_t st)
 {
-    const long long N = (long long)a.nx * a.ny;
-    hipLaunchKernelGGL((get_populations_kernel<S, T>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, a, f);
+    if (j0 < 0 || rows <= 0 || j0 + rows > a.ny) return hipErrorInvalidValue;
+    const long long N = (long long)a.nx * rows;
+    hipLaunchKernelGGL((get_populations_kernel<S, T>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, a, f, j0,
+                       rows);
     return hipGetLastError();
 }
 template <class S, class T> hipError_t launch_ghost_from_state(const StepArgs<S, T> &a, T *g, hipStream_t st)
@@ -1390,7 +1395,7 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_inlet_fold<S, T>(const StepArgs<S, T> &, hipStream_t);            \
     template hipError_t launch_pair_probes<S, T>(const StepArgs<S, T> &, const S *, int, hipStream_t); \
     template hipError_t launch_pair_samples<S, T>(const StepArgs<S, T> &, const S *, hipStream_t); \
-    template hipError_t launch_get_populations<S, T>(const StepArgs<S, T> &, double *, hipStream_t); \
+    template hipError_t launch_get_populations<S, T>(const StepArgs<S, T> &, double *, int, int, hipStream_t); \
     template hipError_t launch_ghost_from_state<S, T>(const StepArgs<S, T> &, T *, hipStream_t);  \
     template hipError_t launch_ghost_from_pops<S, T>(const StepArgs<S, T> &, T *, hipStream_t);
 MRLBM_INSTANTIATE(double, double)

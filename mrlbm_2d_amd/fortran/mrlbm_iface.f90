@@ -106,6 +106,12 @@ module mrlbm_iface
             type(c_ptr), value :: h
             real(c_double), intent(out) :: f(*)
         end function
+        ! same populations without any allocation (callers that cannot allocate device memory)
+        integer(c_int) function mrlbm_get_populations_noalloc(h, f) bind(C, name='mrlbm_get_populations_noalloc')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: f(*)
+        end function
         integer(c_int) function mrlbm_get_forces(h, iters, fd, fl, nmax, cnt) bind(C, name='mrlbm_get_forces')
             import :: c_int, c_ptr, c_int32_t, c_double
             type(c_ptr), value :: h
