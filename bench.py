@@ -7,11 +7,16 @@ and plain FP64.  A "step" is one lattice update of every node; the default path 
 two steps per launch of the bulk kernel (temporal blocking), bit-identical to single steps.
 
 Prints ONE JSON line.  `roofline` prices the dominant kernel against MI355X's 8 TB/s HBM:
-achieved = algorithmic bytes per launch (12 moment words per node and step: 6 read + 6
-written, two steps) / mean launch time, from a HIP event pair around each launch on the
-bulk kernel's own stream;
-traffic = the launch's measured HBM bytes (PMC), below the algorithmic bytes because the
-intermediate step never leaves the chip.  Also: the 512x128 drag adjoint (adjoint / forward
+achieved = algorithmic bytes of the timed launches (12 moment words per node and step: 6
+read + 6 written; the library reports how many steps the timed launches advanced) / their
+summed duration, from a HIP event pair given to each timed launch on the bulk kernel's own
+stream (every launch when the run is short, every 4th otherwise);
+traffic = one launch's HBM bytes from rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE) of the
+committed profile named in `traffic_source`, not measured in this run; below the algorithmic
+bytes because the intermediate step never leaves the chip.
+`fp32_vs_fp64` is the accuracy condition of the FP32 figure: both forms from the same seeded
+state over 100,000 steps (mean Cd, rms Cl, spectral St over the last 50,000; velocity L2 at
+the end), run before the timed legs.  Also: the 512x128 drag adjoint (adjoint / forward
 time) and 256 batched runs (aggregate MLUPS).
 `cpu_baseline` times the oracle (the CPU restatement of the reference step) on a bounded
 number of steps of the same grid.  The reference program itself is not timed here: its
@@ -37,10 +42,13 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1);
 # the summary and the commands are in profiles/bulk2_kernel_8192x2048_hbm_bytes.md
 TRAFFIC_BYTES = {"f32": 951_367_036, "f64": 2_289_709_604}
+TRAFFIC_SOURCE = "profiles/bulk2_kernel_8192x2048_hbm_bytes.md"
 NX, NY = 8192, 2048
-# HIP event pairs around every 4th bulk launch: the per-launch mean without the events' own
-# cost (about 2% of the step when recorded around every launch)
+# HIP event pairs around every 4th bulk launch on long runs: the per-launch mean without the
+# events' own cost (about 2% of the step when recorded around every launch); runs of up to
+# TIME_ALL_UP_TO steps time every launch
 TIMED_EVERY = 4
+TIME_ALL_UP_TO = 400
 XC, YC, RC = 1024.0, 1025.0, 64.0
 UMAX, RE = 0.05, 200.0
 
@@ -70,38 +78,60 @@ def time_form(precision, steps, warmup, **kw):
     lat = make_lattice(precision, **kw)
     lat.step(2, warmup)
     lat.synchronize()
-    lat.set_timing(TIMED_EVERY)
+    lat.set_timing(1 if steps <= TIME_ALL_UP_TO else TIMED_EVERY)
     t0 = time.perf_counter()
     lat.step(2 + warmup, steps)
     lat.synchronize()
     wall = time.perf_counter() - t0
-    step_ms = lat.last_step_ms()
-    launch_ms, launches = lat.last_bulk_ms()
+    t = lat.last_timing()
     assert lat.check_finite() == 0, "non-finite state after the timed steps"
     info = lat.info
     lat.close()
     n = info.nx * info.ny
     mlups = n * steps / (wall * 1e6)
-    # launch_ms: mean duration of one bulk launch, from HIP event pairs around every
-    # TIMED_EVERY-th launch on the bulk kernel's own stream (launches timed = ceil(all launches
-    # / TIMED_EVERY)); a two-step launch covers two steps
-    steps_per_launch = round(steps / (launches * TIMED_EVERY))
+    # the timed launches advanced t.timed_steps steps (2 per two-step launch, 1 per single-step
+    # launch), t.timed_ms in all
     bytes_per_step = int(info.bytes_per_step)
-    achieved = steps_per_launch * bytes_per_step / (launch_ms * 1e-3) / 1e9
-    return dict(mlups=mlups, wall_s=wall, ms_per_step=1e3 * wall / steps, step_ms=step_ms,
-                launch_ms=launch_ms, steps_per_launch=steps_per_launch, bytes_per_step=bytes_per_step,
-                achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS)
+    achieved = t.timed_steps * bytes_per_step / (t.timed_ms * 1e-3) / 1e9
+    return dict(mlups=mlups, wall_s=wall, ms_per_step=1e3 * wall / steps, step_ms=t.step_ms,
+                launch_ms=t.bulk_mean_ms, launch_median_ms=t.bulk_median_ms, launch_min_ms=t.bulk_min_ms,
+                launch_max_ms=t.bulk_max_ms, timed_launches=t.timed_launches, bulk_launches=t.bulk_launches,
+                steps_per_launch=t.bulk_steps / max(t.bulk_launches, 1), timed_steps=t.timed_steps,
+                bytes_per_step=bytes_per_step, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS)
 
 
 def roofline(r, kernel, traffic):
-    """Roofline of the dominant kernel: algorithmic bytes of one launch (two steps of 12 moment
-    words per node) over its mean launch time, against 8 TB/s; traffic = measured HBM bytes
-    of one launch (PMC)."""
+    """Roofline of the dominant kernel: algorithmic bytes of the timed launches (12 moment
+    words per node and step) over their summed duration, against 8 TB/s; traffic = HBM bytes
+    of one launch from the committed PMC profile (None for a launch shape not profiled)."""
     return {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(r["frac"], 4), "traffic": traffic, "kernel": kernel,
-            "steps_per_launch": r["steps_per_launch"],
-            "algorithmic_bytes_per_launch": r["steps_per_launch"] * r["bytes_per_step"],
-            "launch_ms": round(r["launch_ms"], 5)}
+            "frac": round(r["frac"], 4), "traffic": traffic,
+            "traffic_source": (TRAFFIC_SOURCE + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; not measured in "
+                               "this run)") if traffic else None,
+            "kernel": kernel, "steps_per_launch": r["steps_per_launch"],
+            "algorithmic_bytes_per_launch": int(round(r["steps_per_launch"] * r["bytes_per_step"])),
+            "launch_ms": round(r["launch_ms"], 5), "launch_median_ms": round(r["launch_median_ms"], 5),
+            "launch_min_ms": round(r["launch_min_ms"], 5), "launch_max_ms": round(r["launch_max_ms"], 5),
+            "timed_launches": r["timed_launches"], "bulk_launches": r["bulk_launches"],
+            "timed_steps": r["timed_steps"]}
+
+
+def precision_gate(steps, window):
+    """FP32 deviation form against FP64 on the benchmark case from the same seeded state:
+    force series over the last `window` of `steps` steps, velocity field at the end."""
+    from mrlbm_2d_amd import compare
+    res = compare.compare(NX, NY, XC, YC, RC, RE, UMAX, steps, window, seed=3)
+    out = {"steps": steps, "window": window,
+           "cd_rel_err": res["cd_rel_err"], "cl_rms_rel_err": res["cl_rms_rel_err"],
+           "st_spectral_rel_err": res["st_spectral_rel_err"], "l2_u_rel": res["l2_u_rel"],
+           "fd_series_max_rel": res["fd_series_max_rel"],
+           "cd_fp64": res["fp64"]["cd_mean"], "cd_fp32": res["fp32"]["cd_mean"],
+           "cl_rms_fp64": res["fp64"]["cl_rms"], "cl_rms_fp32": res["fp32"]["cl_rms"],
+           "st_spectral_fp64": res["fp64"]["st_spectral"], "st_spectral_fp32": res["fp32"]["st_spectral"],
+           "finite": bool(res["fp64"]["finite"] and res["fp32"]["finite"]),
+           "threshold": 1e-3}
+    out["pass"] = bool(out["finite"] and out["cd_rel_err"] < 1e-3)
+    return out
 
 
 def cpu_baseline(max_steps=10, budget_s=8.0):
@@ -232,6 +262,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--gate-steps", type=int, default=100000,
+                    help="steps of the FP32-vs-FP64 accuracy run (0 skips it)")
+    ap.add_argument("--gate-window", type=int, default=50000)
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--time-blocking", type=int, default=-1,
                     help="-1 auto (two steps per bulk pass), 0 one step per pass, 2..16 two-step strip rows")
@@ -246,6 +279,12 @@ def main():
         copy_calibration(args.copy_calibration, max(args.steps, 1))
         return
     from mrlbm_2d_amd import capi
+    gate = None
+    if args.gate_steps > 0:
+        print(f"[bench] FP32 vs FP64 accuracy run {NX}x{NY}: {args.gate_steps} steps, window {args.gate_window}",
+              file=sys.stderr, flush=True)
+        gate = precision_gate(args.gate_steps, min(args.gate_window, args.gate_steps))
+        print(f"[bench] FP32 vs FP64: {gate}", file=sys.stderr, flush=True)
     print(f"[bench] FP32 deviation form {NX}x{NY}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
     kw = dict(strip_rows=args.strip_rows, time_blocking=args.time_blocking)
@@ -276,7 +315,10 @@ def main():
     else:
         k32, k64 = f"bulk2_kernel<float,float,{tb},false>", f"bulk2_kernel<double,double,{tb},false>"
     out = {
-        "metric": "MLUPS of the D2Q9 regularized LBM step, FP32 deviation form (FP64 beside it)",
+        # the headline metric: MLUPS of the FP32 deviation form (value), its % of HBM peak
+        # (roofline), the FP64 form beside it, the |dCd|/Cd condition (fp32_vs_fp64) and the
+        # adjoint / forward time (adjoint)
+        "metric": "MLUPS + % HBM peak, FP32 vs FP64 at |\u0394Cd|/Cd<1e-3; adjoint/forward time",
         "value": round(r32["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -294,6 +336,7 @@ def main():
         "fp64": {"value": round(r64["mlups"], 1), "ms_per_step": round(r64["ms_per_step"], 5),
                  "roofline": roofline(r64, k64, TRAFFIC_BYTES["f64"] if default_shape else None)},
         "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3),
+        "fp32_vs_fp64": gate,
         "adjoint": adj,
         "batched": batch,
         "cpu_baseline": cpu,

@@ -267,6 +267,21 @@ int mrlbm_last_step_ms(const mrlbm_handle *h, double *ms);
  * launches (at most 4096); *launches = number of launches timed.
  * New: no counterpart in the reference (its loop prints MLUPS only, 2d_mrlbm.f90:540-553). */
 int mrlbm_last_bulk_ms(const mrlbm_handle *h, double *ms_per_launch, int32_t *launches);
+/* Bulk launches of the last mrlbm_step call: all of them, and those timed in timing mode.
+ * Two-step launches advance 2 steps, single-step launches 1; timed_steps counts the steps
+ * the timed launches advanced, so timed_steps * bytes per step / timed_ms is exact whatever
+ * mix of launches ran.  New: no counterpart in the reference. */
+typedef struct mrlbm_timing {
+    double step_ms;               /* mean time per step of the call (HIP events on the handle's stream) */
+    double timed_ms;              /* summed durations of the timed bulk launches */
+    double bulk_mean_ms, bulk_median_ms, bulk_min_ms, bulk_max_ms; /* per timed launch */
+    int32_t bulk_launches;        /* bulk launches of the call */
+    int32_t bulk_steps;           /* steps those launches advanced */
+    int32_t timed_launches;
+    int32_t timed_steps;
+    int32_t reserved[4];
+} mrlbm_timing;
+int mrlbm_last_timing(const mrlbm_handle *h, mrlbm_timing *out);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,7 @@ EXPORTS = (
     "mrlbm_get_surface_p", "mrlbm_get_surface_p_series", "mrlbm_get_rho_inf_mean",
     "mrlbm_get_probes", "mrlbm_get_stats", "mrlbm_reset_stats", "mrlbm_max_ux",
     "mrlbm_check_finite", "mrlbm_set_timing", "mrlbm_last_step_ms", "mrlbm_last_bulk_ms",
+    "mrlbm_last_timing",
     "mrlbm_set_brinkman", "mrlbm_default_adjoint_opts", "mrlbm_adjoint_drag",
     "mrlbm_default_batch_opts", "mrlbm_batch_create", "mrlbm_batch_destroy", "mrlbm_batch_last_error",
     "mrlbm_batch_run", "mrlbm_batch_autotune", "mrlbm_batch_step", "mrlbm_batch_forces",
@@ -59,6 +60,13 @@ class StepOpts(C.Structure):
         ("nprobe", C.c_int32), ("probe_i", C.c_int32 * 4), ("probe_j", C.c_int32 * 4),
         ("reserved", C.c_int32 * 6),
     ]
+
+
+class Timing(C.Structure):
+    _fields_ = [("step_ms", C.c_double), ("timed_ms", C.c_double), ("bulk_mean_ms", C.c_double),
+                ("bulk_median_ms", C.c_double), ("bulk_min_ms", C.c_double), ("bulk_max_ms", C.c_double),
+                ("bulk_launches", C.c_int32), ("bulk_steps", C.c_int32), ("timed_launches", C.c_int32),
+                ("timed_steps", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Info(C.Structure):
@@ -151,6 +159,7 @@ def load(path: str = LIB_PATH):
     L.mrlbm_set_timing.argtypes = [vp, C.c_int32]
     L.mrlbm_last_step_ms.argtypes = [vp, dp]
     L.mrlbm_last_bulk_ms.argtypes = [vp, dp, C.POINTER(C.c_int32)]
+    L.mrlbm_last_timing.argtypes = [vp, C.POINTER(Timing)]
     L.mrlbm_set_brinkman.argtypes = [vp, dp]
     L.mrlbm_default_adjoint_opts.argtypes = [C.POINTER(AdjointOpts)]
     L.mrlbm_default_adjoint_opts.restype = None
@@ -383,6 +392,12 @@ class Lattice:
         v, n = C.c_double(), C.c_int32()
         self._check(self._L.mrlbm_last_bulk_ms(self._h, C.byref(v), C.byref(n)))
         return v.value, n.value
+
+    def last_timing(self) -> Timing:
+        """Bulk launches of the last step call: all, timed, and the steps each set advanced."""
+        t = Timing()
+        self._check(self._L.mrlbm_last_timing(self._h, C.byref(t)))
+        return t
 
 
 def default_batch_opts() -> BatchOpts:

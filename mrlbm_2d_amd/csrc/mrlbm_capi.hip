@@ -720,6 +720,8 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
     int ntimed = 0, nlaunch = 0;
     h->last_ms = h->last_bulk_ms = 0.0;
     h->last_bulk_n = 0;
+    h->tstat = mrlbm_timing{};
+    std::vector<int> timed_steps; // steps advanced by each timed launch (1 or 2)
     if (h->timing) {
         const size_t want = 2 * (size_t)std::min(nsteps / h->timing + 1, kMaxTimed);
         while (h->tev.size() < want) {
@@ -730,28 +732,31 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         HIPCHK(hipEventRecord(h->e0, h->st));
     }
     // the event pair of the next bulk launch (null when it is not timed)
-    auto timed_pair = [&](hipEvent_t &t0, hipEvent_t &t1) {
+    auto timed_pair = [&](hipEvent_t &t0, hipEvent_t &t1, int steps) {
         t0 = t1 = nullptr;
         if (h->timing && nlaunch % h->timing == 0 && ntimed < kMaxTimed) {
             t0 = h->tev[2 * ntimed];
             t1 = h->tev[2 * ntimed + 1];
             ntimed++;
+            timed_steps.push_back(steps);
         }
         nlaunch++;
+        h->tstat.bulk_launches++;
+        h->tstat.bulk_steps += steps;
     };
     // A bulk launch ends in its own stop event (a timing event, else evK): the side work of
     // the next step waits on it, with no marker packet between two bulk launches.  The wait
     // on the previous launch's event is enqueued before the next launch reuses evK.
     auto launch_bulk_pair = [&](const StepArgs<S, T> &bulk) -> hipError_t {
         hipEvent_t t0, t1;
-        timed_pair(t0, t1);
+        timed_pair(t0, t1, 2);
         if (!t1) t1 = h->evK;
         h->bulk_done = t1;
         return launch_bulk2<S, T>(bulk, h->ncb2 * h->nstrips2, h->tb_H, h->st, t0, t1);
     };
     auto launch_bulk_one = [&](const StepArgs<S, T> &a, int nb) -> hipError_t {
         hipEvent_t t0, t1;
-        timed_pair(t0, t1);
+        timed_pair(t0, t1, 1);
         if (!t1) t1 = h->evK;
         h->bulk_done = t1;
         return launch_bulk<S, T>(a, nb, h->st, t0, t1);
@@ -831,12 +836,25 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         HIPCHK(hipEventElapsedTime(&ms, h->e0, h->e1));
         h->last_ms = nsteps > 0 ? ms / nsteps : 0.0;
         double tot = 0;
+        std::vector<double> d(ntimed);
         for (int k = 0; k < ntimed; k++) {
             HIPCHK(hipEventElapsedTime(&ms, h->tev[2 * k], h->tev[2 * k + 1]));
+            d[k] = ms;
             tot += ms;
+            h->tstat.timed_steps += timed_steps[k];
         }
         h->last_bulk_n = ntimed;
         h->last_bulk_ms = ntimed > 0 ? tot / ntimed : 0.0;
+        h->tstat.step_ms = h->last_ms;
+        h->tstat.timed_launches = ntimed;
+        h->tstat.timed_ms = tot;
+        if (ntimed > 0) {
+            std::sort(d.begin(), d.end());
+            h->tstat.bulk_mean_ms = tot / ntimed;
+            h->tstat.bulk_median_ms = ntimed % 2 ? d[ntimed / 2] : 0.5 * (d[ntimed / 2 - 1] + d[ntimed / 2]);
+            h->tstat.bulk_min_ms = d.front();
+            h->tstat.bulk_max_ms = d.back();
+        }
     }
     drain_records(h, rec_iters, opts.nprobe, opts.record_surface_p);
     h->last_nprobe = opts.nprobe;
@@ -1079,6 +1097,13 @@ extern "C" int mrlbm_last_step_ms(const mrlbm_handle *h, double *ms)
 {
     if (!h || !ms) return MRLBM_E_ARG;
     *ms = h->last_ms;
+    return 0;
+}
+
+extern "C" int mrlbm_last_timing(const mrlbm_handle *h, mrlbm_timing *out)
+{
+    if (!h || !out) return MRLBM_E_ARG;
+    *out = h->tstat;
     return 0;
 }
 

@@ -48,6 +48,7 @@ struct mrlbm_handle {
     std::vector<hipEvent_t> tev;
     double last_bulk_ms = 0;
     int last_bulk_n = 0;
+    mrlbm_timing tstat{}; // bulk launches of the last mrlbm_step call
     void *S[2] = {nullptr, nullptr};
     int cur = 0;
     void *pops = nullptr;

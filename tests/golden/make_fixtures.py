@@ -253,6 +253,88 @@ def run_statistics_case(name="stats_synth", nstep=1500, nbct=24, period=137, see
         print(f"{name}: St={vals['Strouhal number (St)']} Cd={vals['Drag coefficinet (C_D)']} ok", flush=True)
 
 
+def _sha256(*arrays):
+    import hashlib
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.asfortranarray(a, np.float64).tobytes(order="F"))
+    return h.hexdigest()
+
+
+# Vortex shedding that the reference can express: ncy = 20 with L = 3/16/2/2 (400 x 100,
+# centre (71, 50), r_cyl = 10.44), channel walls, Re = 100, uniform inlet uo = 0.05, from
+# a seeded noisy restart state.  Shedding sets in after ~20k steps and has saturated by
+# ~30k (lift amplitude 2.41e-2, period ~1.7k steps).
+SHED = dict(ncy=20, L_front=3, L_back=16, L_top=2, L_bot=2, Re=100.0, uo=0.05, iplot=10 ** 7)
+SHED_STEPS, SHED_WINDOW, SHED_SEED, SHED_STRIDE = 50000, 20000, 21, 5
+
+
+def run_shedding_case(name="shed_re100"):
+    """The reference over SHED_STEPS steps with the statistics window on the last
+    SHED_WINDOW steps, then the reference post-processor (2d_statistics.f90) on the files
+    that run wrote, with input_stat.dat filled from the run (nbct, r_cyl, uo and the
+    measured inlet-density mean, as a user of the reference does).  Stored: the post-
+    processor's St, Cd, Cl_rms, Cp profiles; every SHED_STRIDE-th force record; SHA-256
+    digests of the final rho, ux, uy and f (the whole fields, bit for bit) and a few
+    slices of them; the inlet-density means."""
+    p = dict(BASE)
+    p.update(SHED)
+    statsbegin = 2 + SHED_STEPS - SHED_WINDOW
+    p.update(statsbegin=statsbegin, statsend=10 ** 7)
+    nx, ny = grid_size(p)
+    max_iter = 1 + SHED_STEPS
+    with tempfile.TemporaryDirectory() as d:
+        for sub in ("output", "restart", "data_probe", "data_geo", "data_mean", "plots"):
+            os.makedirs(os.path.join(d, sub))
+        f0, ux0, uy0 = noisy_state(nx, ny, SHED_SEED)
+        write_restart(os.path.join(d, "restart", "restart.dat"), 1, f0, ux0, uy0, np.ones((nx, ny)))
+        write_input(os.path.join(d, "input.dat"), p, max_iter, 1)
+        r = subprocess.run([REF_BIN], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name}: reference failed\n{r.stdout}\n{r.stderr}")
+        it, f, ux, uy, rho = read_restart(os.path.join(d, "restart", "restart.dat"), nx, ny)
+        assert it == max_iter, (it, max_iter)
+        params = {}
+        for line in open(os.path.join(d, "simulation_parameters.dat")):
+            key, _, val = line.strip().rpartition(" ")
+            params[key.strip()] = val
+        r_cyl = float(params["Maximum_radius"])
+        nbct = int(params["No. of points on cylinder:"])
+        fr = np.atleast_2d(np.loadtxt(os.path.join(d, "data_mean", "forces_time.dat")))
+        assert fr.shape[0] == SHED_WINDOW and int(fr[0, 0]) == statsbegin, fr[:2]
+        rm = np.atleast_2d(np.loadtxt(os.path.join(d, "rho_mean_inlet.dat")))
+        rho_infty = float(rm[-1, 1])
+        ps = read_unformatted(os.path.join(d, "data_mean", "p_surface_time.dat"))
+        pf = read_unformatted(os.path.join(d, "data_mean", "p_surface_smooth_time.dat"))
+        with open(os.path.join(d, "input_stat.dat"), "w") as fh:
+            fh.write(f"&parameters\n\tnbct = {nbct}\n\tr_cy = {r_cyl!r}\n\tuo = {p['uo']!r}\n"
+                     f"\trho_infty = {rho_infty!r}\n/\n")
+        r = subprocess.run([STATS_BIN], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(r.stdout + r.stderr)
+        vals = {}
+        for line in open(os.path.join(d, "plots", "drag_lift.dat")):
+            key, _, v = line.rpartition(":")
+            vals[key.strip()] = float(v)
+        ncycle = [l for l in r.stdout.splitlines() if "cycle" in l]
+        sel = np.arange(0, SHED_WINDOW, SHED_STRIDE)
+        out = dict(
+            nx=nx, ny=ny, nsteps=SHED_STEPS, iter_first=2, seed=SHED_SEED, statsbegin=statsbegin,
+            f0_sha256=np.array(_sha256(f0)),
+            fields_sha256=np.array(_sha256(rho, ux, uy)), f_sha256=np.array(_sha256(f)),
+            rho_row=rho[:, ny // 2].copy(), ux_row=ux[:, ny // 2].copy(), uy_col=uy[nx // 4, :].copy(),
+            r_cyl=r_cyl, nbct=nbct, rho_infty=rho_infty, rho_mean_inlet=rm,
+            force_stride=SHED_STRIDE, forces=fr[sel], theta=np.loadtxt(os.path.join(d, "data_mean", "theta.dat")),
+            ps_last=ps[-1][1], pfit_last=pf[-1][1], ps_iter=np.array([ps[-1][0]]),
+            st=vals["Strouhal number (St)"], cd=vals["Drag coefficinet (C_D)"], cl_rms=vals["Lift coefficinet (C_L"],
+            cp=np.loadtxt(os.path.join(d, "plots", "cp.dat")), cp_fit=np.loadtxt(os.path.join(d, "plots", "cp_fit.dat")),
+            stats_stdout=np.array("\n".join(ncycle)),
+            params_json=np.array(json.dumps(dict(p, kind="shedding"))))
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+        print(f"{name}: {nx}x{ny} steps={SHED_STEPS} nbct={nbct} St={out['st']} Cd={out['cd']} "
+              f"Cl_rms={out['cl_rms']} ({ncycle}) ok", flush=True)
+
+
 def main(names=None):
     if not os.path.exists(REF_BIN):
         subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle")], check=True)
@@ -262,6 +344,8 @@ def main(names=None):
         run_case(name, ov, n, seed)
     if not names or "stats_synth" in names:
         run_statistics_case()
+    if not names or "shed_re100" in names:
+        run_shedding_case()
 
 
 if __name__ == "__main__":
