@@ -50,6 +50,9 @@ NX, NY = 8192, 2048
 TIMED_EVERY = 4
 TIME_ALL_UP_TO = 400
 XC, YC, RC = 1024.0, 1025.0, 64.0
+# oracle MLUPS / reference MLUPS measured in the build container (8 cores) on the reference's
+# own 400x100 (ncy = 20, L = 3/16/2/2) and 920x420 (its input.dat) cases, post_process off
+REF_RATIO_8T, REF_RATIO_1T = (3.52, 3.73), (1.29, 2.15)
 UMAX, RE = 0.05, 200.0
 
 
@@ -74,18 +77,48 @@ def make_lattice(precision, nx=NX, ny=NY, strip_rows=0, kernel=0, time_blocking=
     return lat
 
 
-def time_form(precision, steps, warmup, **kw):
+def run_form(precision, steps, warmup, gate_steps=0, gate_window=0, **kw):
+    """One 8192 x 2048 run in one precision form.  With gate_steps > 0 it first advances
+    gate_steps steps recording the force series over the last gate_window (the FP32-vs-FP64
+    accuracy comparison); then `warmup` untimed and `steps` timed steps continue the same run
+    (the timed steps are steps of the run whose accuracy is checked); the velocity field at
+    the end."""
+    from mrlbm_2d_amd import capi
     lat = make_lattice(precision, **kw)
-    lat.step(2, warmup)
+    it = 2
+    fd, fl = [], []
+    if gate_steps > 0:
+        o = capi.default_step_opts()
+        o.post_process = 1
+        o.statsbegin = 2 + gate_steps - gate_window
+        o.statsend = 1 + gate_steps
+        chunk = 2000
+        while it <= 1 + gate_steps:
+            n = min(chunk, 2 + gate_steps - it)
+            lat.step(it, n, o)
+            _, a, b = lat.forces()
+            fd.append(a)
+            fl.append(b)
+            it += n
+            if (it - 2) % 20000 == 0:
+                print(f"[bench] precision {precision}: {it - 2} of {gate_steps} accuracy steps", file=sys.stderr,
+                      flush=True)
+    lat.step(it, warmup)
+    it += warmup
     lat.synchronize()
     lat.set_timing(1 if steps <= TIME_ALL_UP_TO else TIMED_EVERY)
     t0 = time.perf_counter()
-    lat.step(2 + warmup, steps)
+    lat.step(it, steps)
     lat.synchronize()
     wall = time.perf_counter() - t0
     t = lat.last_timing()
-    assert lat.check_finite() == 0, "non-finite state after the timed steps"
+    finite = lat.check_finite() == 0
+    assert finite, "non-finite state after the timed steps"
     info = lat.info
+    out = {}
+    if gate_steps > 0:
+        _, ux, uy = lat.fields()
+        out = dict(fd=np.concatenate(fd), fl=np.concatenate(fl), ux=ux, uy=uy, finite=finite, r_cyl=info.r_cyl)
     lat.close()
     n = info.nx * info.ny
     mlups = n * steps / (wall * 1e6)
@@ -97,7 +130,8 @@ def time_form(precision, steps, warmup, **kw):
                 launch_ms=t.bulk_mean_ms, launch_median_ms=t.bulk_median_ms, launch_min_ms=t.bulk_min_ms,
                 launch_max_ms=t.bulk_max_ms, timed_launches=t.timed_launches, bulk_launches=t.bulk_launches,
                 steps_per_launch=t.bulk_steps / max(t.bulk_launches, 1), timed_steps=t.timed_steps,
-                bytes_per_step=bytes_per_step, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS)
+                bytes_per_step=bytes_per_step, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
+                first_timed_step=it), out
 
 
 def roofline(r, kernel, traffic):
@@ -116,20 +150,25 @@ def roofline(r, kernel, traffic):
             "timed_steps": r["timed_steps"]}
 
 
-def precision_gate(steps, window):
-    """FP32 deviation form against FP64 on the benchmark case from the same seeded state:
-    force series over the last `window` of `steps` steps, velocity field at the end."""
+def precision_gate(run64, run32, steps, window):
+    """FP32 deviation form against FP64 from the same seeded state (the two runs of run_form):
+    mean Cd, rms Cl and the lift spectrum's St over the window, velocity L2 at the end."""
     from mrlbm_2d_amd import compare
-    res = compare.compare(NX, NY, XC, YC, RC, RE, UMAX, steps, window, seed=3)
+    s64, s32 = compare.summarize(run64, UMAX), compare.summarize(run32, UMAX)
+    du = np.sqrt(np.sum((run64["ux"] - run32["ux"]) ** 2 + (run64["uy"] - run32["uy"]) ** 2))
+    nu = np.sqrt(np.sum(run64["ux"] ** 2 + run64["uy"] ** 2))
     out = {"steps": steps, "window": window,
-           "cd_rel_err": res["cd_rel_err"], "cl_rms_rel_err": res["cl_rms_rel_err"],
-           "st_spectral_rel_err": res["st_spectral_rel_err"], "l2_u_rel": res["l2_u_rel"],
-           "fd_series_max_rel": res["fd_series_max_rel"],
-           "cd_fp64": res["fp64"]["cd_mean"], "cd_fp32": res["fp32"]["cd_mean"],
-           "cl_rms_fp64": res["fp64"]["cl_rms"], "cl_rms_fp32": res["fp32"]["cl_rms"],
-           "st_spectral_fp64": res["fp64"]["st_spectral"], "st_spectral_fp32": res["fp32"]["st_spectral"],
-           "finite": bool(res["fp64"]["finite"] and res["fp32"]["finite"]),
-           "threshold": 1e-3}
+           "cd_rel_err": abs(s32["cd_mean"] - s64["cd_mean"]) / abs(s64["cd_mean"]),
+           "cl_rms_rel_err": abs(s32["cl_rms"] - s64["cl_rms"]) / abs(s64["cl_rms"]),
+           "st_spectral_rel_err": abs(s32["st_spectral"] - s64["st_spectral"]) / s64["st_spectral"],
+           "l2_u_rel": float(du / nu),
+           "fd_series_max_rel": float(np.max(np.abs(run64["fd"] - run32["fd"])) / np.max(np.abs(run64["fd"]))),
+           "cd_fp64": s64["cd_mean"], "cd_fp32": s32["cd_mean"],
+           "cl_rms_fp64": s64["cl_rms"], "cl_rms_fp32": s32["cl_rms"],
+           "st_spectral_fp64": s64["st_spectral"], "st_spectral_fp32": s32["st_spectral"],
+           "finite": bool(run64["finite"] and run32["finite"]),
+           "threshold": 1e-3,
+           "velocity_field_at": "the end of the timed steps, both forms at the same step"}
     out["pass"] = bool(out["finite"] and out["cd_rel_err"] < 1e-3)
     return out
 
@@ -159,7 +198,11 @@ def cpu_baseline(max_steps=10, budget_s=8.0):
     return dict(value=round(NX * NY / (per_step * 1e6), 3), unit="MLUPS", cores=threads, kind="port",
                 sample=f"oracle (C restatement of the reference step, gcc -O2, OpenMP per-node sweeps on "
                        f"{threads} threads, boundary work serial as in the reference) on the same 8192x2048 "
-                       f"grid, {len(times)} timed steps after one untimed step, {per_step:.3f} s/step")
+                       f"grid, {len(times)} timed steps after one untimed step, {per_step:.3f} s/step; "
+                       f"against the reference itself (flang -O3 -fopenmp build of 2d_mrlbm.f90, serial "
+                       f"streaming) on 400x100 and 920x420 in the build container the oracle ran "
+                       f"{REF_RATIO_8T[0]}-{REF_RATIO_8T[1]}x faster on 8 threads and "
+                       f"{REF_RATIO_1T[0]}-{REF_RATIO_1T[1]}x on 1 thread")
 
 
 def adjoint_bench():
@@ -279,20 +322,21 @@ def main():
         copy_calibration(args.copy_calibration, max(args.steps, 1))
         return
     from mrlbm_2d_amd import capi
-    gate = None
-    if args.gate_steps > 0:
-        print(f"[bench] FP32 vs FP64 accuracy run {NX}x{NY}: {args.gate_steps} steps, window {args.gate_window}",
-              file=sys.stderr, flush=True)
-        gate = precision_gate(args.gate_steps, min(args.gate_window, args.gate_steps))
-        print(f"[bench] FP32 vs FP64: {gate}", file=sys.stderr, flush=True)
-    print(f"[bench] FP32 deviation form {NX}x{NY}, warmup {args.warmup}, steps {args.steps}",
-          file=sys.stderr, flush=True)
+    gate_steps = max(args.gate_steps, 0)
+    gate_window = min(args.gate_window, gate_steps)
     kw = dict(strip_rows=args.strip_rows, time_blocking=args.time_blocking)
-    r32 = time_form(capi.MRLBM_FP32_DEV, args.steps, args.warmup, **kw)
-    print(f"[bench] FP32: {r32}", file=sys.stderr, flush=True)
-    print(f"[bench] FP64 plain form {NX}x{NY}", file=sys.stderr, flush=True)
-    r64 = time_form(capi.MRLBM_FP64, args.steps, args.warmup, **kw)
+    # FP64 first, then FP32 (the headline): each form's timed steps continue its own run
+    # right after the accuracy steps
+    print(f"[bench] FP64 plain form {NX}x{NY}: {gate_steps} accuracy steps (window {gate_window}), warmup "
+          f"{args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
+    r64, run64 = run_form(capi.MRLBM_FP64, args.steps, args.warmup, gate_steps, gate_window, **kw)
     print(f"[bench] FP64: {r64}", file=sys.stderr, flush=True)
+    print(f"[bench] FP32 deviation form {NX}x{NY}: same", file=sys.stderr, flush=True)
+    r32, run32 = run_form(capi.MRLBM_FP32_DEV, args.steps, args.warmup, gate_steps, gate_window, **kw)
+    print(f"[bench] FP32: {r32}", file=sys.stderr, flush=True)
+    gate = precision_gate(run64, run32, gate_steps, gate_window) if gate_steps > 0 else None
+    del run64, run32
+    print(f"[bench] FP32 vs FP64: {gate}", file=sys.stderr, flush=True)
     adj = None
     if not args.no_adjoint:
         print("[bench] drag adjoint, 512x128", file=sys.stderr, flush=True)
@@ -337,6 +381,8 @@ def main():
                  "roofline": roofline(r64, k64, TRAFFIC_BYTES["f64"] if default_shape else None)},
         "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3),
         "fp32_vs_fp64": gate,
+        # the timed steps continue each form's accuracy run from this step number
+        "first_timed_step": r32["first_timed_step"],
         "adjoint": adj,
         "batched": batch,
         "cpu_baseline": cpu,

@@ -50,6 +50,8 @@ struct orc_state {
     int *idx; /* 0-based sorted order */
     double *cxp, *cyp, *Hxxp, *Hyyp, *Hxyp; /* [9*k + q] rotated tables at boundary nodes */
     double *uxp_b, *uyp_b, *Fd, *Fl, *p_cy, *ps_cy, *p_fit;
+    double *uin; /* inlet velocity per row (uniform uo, or the parabolic extension) */
+    double *chi; /* Brinkman solid fraction (extension), or NULL */
     int Incs[16][Q], Outs[16][Q];
     /* recorded outputs */
     double F_drag_net, F_lift_net, rho_inf_mean;
@@ -89,9 +91,15 @@ static void get_coord(orc_state *s)
 {
     const int nx = s->nx, ny = s->ny;
     const double epsilon = (double)0.5f;
-    s->x_c = (double)(float)(s->nfront + (s->p.ncy / 2) + 1); /* M:946 */
-    s->y_c = (double)(float)(s->nbot + s->p.ncy / 2);         /* M:947 */
-    s->r_c = (double)(float)(s->p.ncy / 2);                   /* M:948 */
+    if (s->p.geometry_explicit) { /* extension: the centre and radius as given */
+        s->x_c = s->p.x_c;
+        s->y_c = s->p.y_c;
+        s->r_c = s->p.r_c;
+    } else {
+        s->x_c = (double)(float)(s->nfront + (s->p.ncy / 2) + 1); /* M:946 */
+        s->y_c = (double)(float)(s->nbot + s->p.ncy / 2);         /* M:947 */
+        s->r_c = (double)(float)(s->p.ncy / 2);                   /* M:948 */
+    }
     for (int i = 0; i < nx; i++)
         for (int j = 0; j < ny; j++) { /* M:958-966 */
             const int n = IDX(i, j);
@@ -166,6 +174,27 @@ static void get_coord(orc_state *s)
         if (k == 0 || r > max_radii) max_radii = r;
     }
     s->r_cyl = max_radii;
+    if (s->p.label_fluid) {
+        /* extension: nodes of labels 0/6/9/15 (no masks in the reference, M:2008) leave the
+           list and are fluid; r_cyl and the other nodes' labels stay as computed above */
+        int kept = 0;
+        for (int k = 0; k < nbct; k++) {
+            const int l = s->label[k];
+            if (l == 0 || l == 6 || l == 9 || l == 15) {
+                const int n = IDX(s->bou_i[k], s->bou_j[k]);
+                s->isbound[n] = 0;
+                s->issolid[n] = 0;
+                s->isfluid[n] = 1;
+                continue;
+            }
+            s->bou_i[kept] = s->bou_i[k];
+            s->bou_j[kept] = s->bou_j[k];
+            s->label[kept] = l;
+            kept++;
+        }
+        nbct = kept;
+        s->nbct = kept;
+    }
 
     const double delx = sqrt(2.0); /* M:1055 */
     s->delx = delx;
@@ -218,7 +247,7 @@ orc_state *orc_create(const orc_params *p)
 {
     orc_state *s = (orc_state *)calloc(1, sizeof(orc_state));
     s->p = *p;
-    if (p->channel_with_cylinder) { /* M:86-92 */
+    if (p->channel_with_cylinder && !p->geometry_explicit) { /* M:86-92 */
         s->nfront = p->L_front * p->ncy;
         s->nback = p->L_back * p->ncy;
         s->ntop = p->L_top * p->ncy;
@@ -242,11 +271,22 @@ orc_state *orc_create(const orc_params *p)
 
     s->uinlet = p->uo; /* M:123-125 */
     s->vinlet = 0.0;
+    s->uin = dalloc((size_t)s->ny);
+    for (int j = 0; j < s->ny; j++) {
+        if (p->inlet_parabolic) { /* extension: u_max = uo at mid-channel, 0 at both walls */
+            const double sj = (double)j / (double)(s->ny - 1);
+            s->uin[j] = ((4.0 * p->uo) * sj) * (1.0 - sj);
+        } else {
+            s->uin[j] = s->uinlet; /* M:258 */
+        }
+    }
+    /* Re is defined on the inlet's mean velocity: uo, or 2/3 uo for the parabola */
+    const double uref = p->inlet_parabolic ? (2.0 / 3.0) * p->uo : s->uinlet;
     const float wf[Q] = {16.0f / 36.0f, 4.0f / 36.0f, 4.0f / 36.0f, 4.0f / 36.0f, 4.0f / 36.0f,
                          1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f}; /* M:129, REAL(4) */
     for (int k = 0; k < Q; k++) s->w[k] = (double)wf[k];
-    s->nu = s->uinlet * (double)(s->ny - 1) / p->Re; /* M:133: (uinlet*(ny-1))/Re */
-    if (p->channel_with_cylinder) s->nu = s->uinlet * (2.0 * s->r_cyl) / p->Re;
+    s->nu = uref * (double)(s->ny - 1) / p->Re; /* M:133: (uinlet*(ny-1))/Re */
+    if (p->channel_with_cylinder) s->nu = uref * (2.0 * s->r_cyl) / p->Re; /* M:134 */
     s->omega = 1.0 / ((3.0 * s->nu) + (1.0 / 2.0)); /* M:135 (3.0 is REAL(4), exact) */
     s->cs2 = 1.0 / 3.0;
     for (int k = 0; k < Q; k++) { /* M:171-178 */
@@ -284,7 +324,7 @@ void orc_destroy(orc_state *s)
                     s->ip3, s->jp3, s->ip4, s->jp4, s->xref1, s->yref1, s->xref2, s->yref2, s->xref3,
                     s->yref3, s->xref4, s->yref4, s->delta, s->theta_c, s->thetas, s->idx, s->cxp,
                     s->cyp, s->Hxxp, s->Hyyp, s->Hxyp, s->uxp_b, s->uyp_b, s->Fd, s->Fl, s->p_cy,
-                    s->ps_cy, s->p_fit};
+                    s->ps_cy, s->p_fit, s->uin, s->chi};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
     free(s);
 }
@@ -759,7 +799,7 @@ void orc_step(orc_state *s, int iter)
         }
     if (!P->x_periodic)
         for (int j = 0; j < ny; j++) {
-            s->ux[IDX(0, j)] = s->uinlet;
+            s->ux[IDX(0, j)] = s->uin[j];
             s->uy[IDX(0, j)] = s->vinlet;
         }
     if (!P->y_periodic)
@@ -795,6 +835,17 @@ void orc_step(orc_state *s, int iter)
         for (int i = 0; i < nx; i++) apply_edge_bc(s, i, ny - 1, 2, P->incomp_bc);
     if (!P->y_periodic)
         for (int i = 0; i < nx; i++) apply_edge_bc(s, i, 0, 3, P->incomp_bc);
+    /* Brinkman penalization (extension): off the outer ring and off the wall list */
+    if (s->chi)
+#pragma omp parallel for schedule(static)
+        for (int j = 1; j < ny - 1; j++)
+            for (int i = 1; i < nx - 1; i++) {
+                const int n = IDX(i, j);
+                if (s->isbound[n] == 1) continue;
+                const double keep = 1.0 - s->chi[n];
+                s->ux[n] = s->ux[n] * keep;
+                s->uy[n] = s->uy[n] * keep;
+            }
     /* post_process: inlet density mean, solid-u zeroing on plot steps (M:355-387) */
     if (P->post_process) {
         if (iter >= P->statsbegin && iter <= P->statsend) {
@@ -876,6 +927,17 @@ void orc_step(orc_state *s, int iter)
                 if (EY[k] > 0) FK(s->f, IDX(i, 0), k) = FK(s->ftmp, IDX(i, ny - 1), k);
                 if (EY[k] < 0) FK(s->f, IDX(i, ny - 1), k) = FK(s->ftmp, IDX(i, 0), k);
             }
+}
+
+void orc_set_brinkman(orc_state *s, const double *chi)
+{
+    if (!chi) {
+        free(s->chi);
+        s->chi = NULL;
+        return;
+    }
+    if (!s->chi) s->chi = dalloc((size_t)s->N);
+    memcpy(s->chi, chi, sizeof(double) * (size_t)s->N);
 }
 
 void orc_run(orc_state *s, int iter_first, int nsteps)

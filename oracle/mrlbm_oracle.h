@@ -29,6 +29,16 @@ typedef struct orc_params {
     int channel_with_cylinder, channel_with_square;
     int incomp_bc, incomp_bulk, vel_interp, mom_interp, rotated_coordinate;
     int strong_mass_bc, eq_mass_bc, weak_mass_bc;
+    /* Extensions beyond the reference (its cases keep all of these 0):
+     * geometry_explicit: grid nx, ny and the cylinder's 1-based centre (x_c, y_c) and nominal
+     *   radius r_c given directly, instead of derived from ncy and L_* (M:86-92, M:946-948);
+     * inlet_parabolic: inlet u(j) = 4 uo s (1 - s), s = (j - 1)/(ny - 1), in place of the
+     *   uniform uo (M:258), and nu taken on the mean velocity 2/3 uo (M:133-134);
+     * label_fluid: boundary nodes whose quadrant label the reference leaves without masks
+     *   (0, 6, 9, 15; M:2008-2057) are bulk fluid instead of boundary nodes. */
+    int geometry_explicit;
+    double x_c, y_c, r_c;
+    int inlet_parabolic, label_fluid;
 } orc_params;
 
 typedef struct orc_state orc_state;
@@ -57,6 +67,12 @@ void orc_get_fout(const orc_state *s, double *fout);
 /* Post-BC moments of the last step: rho, ux, uy and the post-collision mxx, myy, mxy. */
 void orc_get_moments(const orc_state *s, double *rho, double *ux, double *uy,
                      double *mxx, double *myy, double *mxy);
+
+/* Brinkman solid fraction chi (N values, node i + nx*j; NULL removes it).  Extension, no
+ * counterpart in the reference: after the edge BCs, every node off the domain's outer ring
+ * and off the curved-wall list has its velocity scaled, u <- (1 - chi) u, before the
+ * plot-step zeroing and the collision. */
+void orc_set_brinkman(orc_state *s, const double *chi);
 
 /* One iteration of the main loop body (2d_mrlbm.f90:229-535) numbered `iter`. */
 void orc_step(orc_state *s, int iter);

@@ -54,9 +54,18 @@ class OracleParams:
     strong_mass_bc: int = 1
     eq_mass_bc: int = 0
     weak_mass_bc: int = 0
+    # extensions beyond the reference (all 0 for the reference's own cases): grid and
+    # cylinder centre/radius given directly, parabolic inlet with Re on 2/3 uo, and the
+    # nodes of labels 0/6/9/15 as fluid (oracle/mrlbm_oracle.h)
+    geometry_explicit: int = 0
+    x_c: float = 0.0
+    y_c: float = 0.0
+    r_c: float = 0.0
+    inlet_parabolic: int = 0
+    label_fluid: int = 0
 
 
-_FLOAT_FIELDS = {"Re", "uo"}
+_FLOAT_FIELDS = {"Re", "uo", "x_c", "y_c", "r_c"}
 
 
 class _CParams(C.Structure):
@@ -91,6 +100,7 @@ def lib():
         L.orc_get_fout.argtypes = [vp, dp]
         L.orc_get_moments.argtypes = [vp, dp, dp, dp, dp, dp, dp]
         L.orc_step.argtypes = [vp, C.c_int]
+        L.orc_set_brinkman.argtypes = [vp, dp]
         L.orc_run.argtypes = [vp, C.c_int, C.c_int]
         L.orc_get_forces.argtypes = [vp, dp, dp]
         L.orc_get_surface_p.argtypes = [vp, dp, dp]
@@ -190,6 +200,15 @@ class Oracle:
         a = [np.zeros(self.N) for _ in range(6)]
         lib().orc_get_moments(self._h, *[_dp(x) for x in a])
         return [x.reshape(self.nx, self.ny, order="F") for x in a]
+
+    def set_brinkman(self, chi=None):
+        """Brinkman solid fraction chi (nx, ny) column-major, or None to remove it."""
+        if chi is None:
+            lib().orc_set_brinkman(self._h, C.POINTER(C.c_double)())
+            return
+        a = np.ascontiguousarray(np.asarray(chi, np.float64).reshape(-1, order="F"))
+        assert a.size == self.N
+        lib().orc_set_brinkman(self._h, _dp(a))
 
     def step(self, it: int):
         lib().orc_step(self._h, it)
