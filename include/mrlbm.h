@@ -162,9 +162,11 @@ int mrlbm_get_probes(mrlbm_handle *h, int32_t *iters, double *vals, int32_t max,
 int mrlbm_get_stats(const mrlbm_handle *h, double *mean_ux, double *mean_uy, double *rms_ux,
                     double *rms_uy, int64_t *nsamples);
 int mrlbm_reset_stats(mrlbm_handle *h);
-/* max(ux) over all nodes including solid ones, as logged every 100 steps (:546). */
+/* max(ux) over all nodes including solid ones, as logged every 100 steps (:546); NaN values
+ * are ignored (mrlbm_check_finite reports them).  Reduced on the device. */
 int mrlbm_max_ux(const mrlbm_handle *h, double *value);
-/* MRLBM_E_NONFINITE when any stored moment is NaN/Inf (the reference never checks). */
+/* MRLBM_E_NONFINITE when any stored moment is NaN/Inf (the reference never checks).  Reduced
+ * on the device: one read of the state, 16 bytes back to the host. */
 int mrlbm_check_finite(const mrlbm_handle *h);
 
 /* Brinkman solid fraction chi(nx,ny), column-major (no counterpart in the reference): before

@@ -649,7 +649,7 @@ template <class S, class T> static int adjoint_setup(mrlbm_handle *h, int K, int
 }
 
 template <class S, class T>
-static int adjoint_typed(mrlbm_handle *h, const mrlbm_adjoint_opts *o, double *dJ_dchi, mrlbm_adjoint_result *res)
+static int adjoint_run(mrlbm_handle *h, const mrlbm_adjoint_opts *o, double *dJ_dchi, mrlbm_adjoint_result *res)
 {
     using clock = std::chrono::steady_clock;
     const int nsteps = o->nsteps, window = o->window, K = o->checkpoint_interval;
@@ -838,6 +838,35 @@ extern "C" void mrlbm_default_adjoint_opts(mrlbm_adjoint_opts *o)
     o->nsteps = 4000;
     o->window = 2000;
     o->checkpoint_interval = 200;
+}
+
+// The adjoint's forward sweep steps the handle with its own statistics window: the running
+// inlet-density mean and its pending-sample bookkeeping are saved before and restored after,
+// and a zero Brinkman field installed for the sweep is removed again, so later mrlbm_step
+// calls see the handle as the caller left it (and keep the two-step path).
+template <class S, class T>
+static int adjoint_typed(mrlbm_handle *h, const mrlbm_adjoint_opts *o, double *dJ_dchi, mrlbm_adjoint_result *res)
+{
+    const bool had_chi = h->chi != nullptr;
+    double rho_mean = 0.0;
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemcpy(&rho_mean, h->d_rho_mean, sizeof(double), hipMemcpyDeviceToHost));
+    const int valid = h->prev_inlet_valid, buf = h->prev_inlet_buf;
+    const double c = h->prev_inlet_c;
+    const int rc = adjoint_run<S, T>(h, o, dJ_dchi, res);
+    (void)hipStreamSynchronize(h->st);
+    (void)hipStreamSynchronize(h->st2);
+    const hipError_t e = hipMemcpy(h->d_rho_mean, &rho_mean, sizeof(double), hipMemcpyHostToDevice);
+    h->prev_inlet_valid = valid;
+    h->prev_inlet_buf = buf;
+    h->prev_inlet_c = c;
+    if (!had_chi && h->chi) {
+        const int rc2 = mrlbm_set_brinkman(h, nullptr);
+        if (!rc && rc2) return rc2;
+    }
+    if (rc) return rc;
+    HIPCHK(e);
+    return 0;
 }
 
 extern "C" int mrlbm_adjoint_drag(mrlbm_handle *h, const mrlbm_adjoint_opts *o, double *dJ_dchi,

@@ -148,7 +148,7 @@ template <class S, class T> static int create_typed(mrlbm_handle *h)
     h->d_rho_mean = dmalloc<double>(1, e);
     HIPCHK(e);
     HIPCHK(hipMemset(h->d_rho_mean, 0, sizeof(double)));
-    h->d_red = dmalloc<double>(4096, e);
+    h->d_red = dmalloc<double>(kRedWords, e);
     HIPCHK(e);
     if (h->has_b) {
         const BoundaryTables &t = h->bt;
@@ -1059,14 +1059,30 @@ extern "C" int mrlbm_reset_stats(mrlbm_handle *h)
     return 0;
 }
 
+// Largest ux and the non-finite flag of the current state, reduced on the device into the
+// handle's reduction scratch (out[0], out[1]).
+static int scan_state(mrlbm_handle *h, double out[2])
+{
+    if (h->from_pops) FAIL(MRLBM_E_STATE, "no step since mrlbm_set_populations");
+    HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0)); // the side stream's last writes
+    const hipError_t e = h->f32 ? launch_state_scan<float>((const float *)h->S[h->cur], h->plane, h->nx, h->ny,
+                                                           h->pitch, h->d_red, kRedWords, h->st)
+                                : launch_state_scan<double>((const double *)h->S[h->cur], h->plane, h->nx,
+                                                            h->ny, h->pitch, h->d_red, kRedWords, h->st);
+    HIPCHK(e);
+    HIPCHK(hipMemcpyAsync(out, h->d_red, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
 extern "C" int mrlbm_max_ux(const mrlbm_handle *hc, double *v)
 {
     auto *h = const_cast<mrlbm_handle *>(hc);
     if (!h || !v) return MRLBM_E_ARG;
-    std::vector<double> ux((size_t)h->nx * h->ny);
-    const int rc = mrlbm_get_moments(h, nullptr, ux.data(), nullptr, nullptr, nullptr, nullptr);
+    double r[2];
+    const int rc = scan_state(h, r);
     if (rc) return rc;
-    *v = *std::max_element(ux.begin(), ux.end());
+    *v = r[0];
     return 0;
 }
 
@@ -1074,14 +1090,10 @@ extern "C" int mrlbm_check_finite(const mrlbm_handle *hc)
 {
     auto *h = const_cast<mrlbm_handle *>(hc);
     if (!h) return MRLBM_E_ARG;
-    const size_t N = (size_t)h->nx * h->ny;
-    std::vector<double> m[6];
-    for (auto &v : m) v.resize(N);
-    const int rc = mrlbm_get_moments(h, m[0].data(), m[1].data(), m[2].data(), m[3].data(), m[4].data(), m[5].data());
+    double r[2];
+    const int rc = scan_state(h, r);
     if (rc) return rc;
-    for (auto &v : m)
-        for (double x : v)
-            if (!std::isfinite(x)) FAIL(MRLBM_E_NONFINITE, "non-finite moment in the state");
+    if (r[1] != 0.0) FAIL(MRLBM_E_NONFINITE, "non-finite moment in the state");
     return 0;
 }
 

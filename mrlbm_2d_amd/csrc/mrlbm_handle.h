@@ -25,6 +25,8 @@ template <class T> inline T *dmalloc(size_t n, hipError_t &e)
     e = hipMalloc(&p, n ? n * sizeof(T) : sizeof(T));
     return (T *)p;
 }
+constexpr int kRedWords = 4096;
+
 struct mrlbm_handle {
     mrlbm_params p{};
     std::string err;
@@ -65,7 +67,7 @@ struct mrlbm_handle {
     double *d_rho_mean = nullptr;
     double *d_acc = nullptr;
     int64_t nsamples = 0;
-    double *d_red = nullptr; // reduction scratch
+    double *d_red = nullptr; // reduction scratch of kRedWords doubles
     int prev_inlet_valid = 0, prev_inlet_buf = 0;
     double prev_inlet_c = 0;
     int strip_rows = 32, ncg = 0, ncb = 0, nstrips = 0;

@@ -90,6 +90,12 @@ hipError_t launch_pair_probes(const StepArgs<S, T> &a, const S *s2, int slot2, h
 // rectangles that the two-step bulk writes (a: sub-step 1; s2: state t+2); after both.
 template <class S, class T>
 hipError_t launch_pair_samples(const StepArgs<S, T> &a, const S *s2, hipStream_t st);
+// Largest ux (NaN ignored) and a non-finite flag (1.0 or 0.0) over the six stored moments
+// of the nx x ny nodes of one state, into scratch[0] and scratch[1]; the rest of the
+// scratch_words doubles holds per-block partial results.
+template <class S>
+hipError_t launch_state_scan(const S *src, long long plane, int nx, int ny, int pitch, double *scratch,
+                             int scratch_words, hipStream_t st);
 // Post-stream populations of rows j0 .. j0+rows-1 into a dense band f(nx, rows, 0:8).
 template <class S, class T>
 hipError_t launch_get_populations(const StepArgs<S, T> &a, double *f_band, int j0, int rows, hipStream_t st);

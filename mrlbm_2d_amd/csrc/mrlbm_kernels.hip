@@ -1089,6 +1089,64 @@ __global__ void __launch_bounds__(256) plane_from_dense_kernel(const double *src
     dst[t] = i < nx ? (S)(src[(long long)j * nx + i] - shift) : (S)0;
 }
 
+// State scan for the log step (2d_mrlbm.f90:546) and the finite check: per block, the
+// largest ux over the block's nodes (NaN ignored) and whether any of the six stored moments
+// of its nodes is NaN or Inf; scan_finish_kernel folds the block results.  part holds
+// (max, flag) per block.
+template <class S>
+__global__ void __launch_bounds__(256) state_scan_kernel(const S *src, long long plane, int nx, int ny, int pitch,
+                                                         double *part)
+{
+    __shared__ double smax[4];
+    __shared__ int sbad[4];
+    const long long N = (long long)nx * ny;
+    double mx = -INFINITY;
+    int bad = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx), j = (int)(t / nx);
+        const long long n = (long long)j * pitch + i;
+        const double u = (double)src[plane + n];
+        mx = fmax(mx, u);
+#pragma unroll
+        for (int p = 0; p < 6; p++) bad |= !isfinite((double)src[p * plane + n]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmax(mx, __shfl_down(mx, o, 64));
+        bad |= __shfl_down(bad, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        smax[w] = mx;
+        sbad[w] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+            mx = fmax(mx, smax[k]);
+            bad |= sbad[k];
+        }
+        part[2 * blockIdx.x] = mx;
+        part[2 * blockIdx.x + 1] = bad ? 1.0 : 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(64) scan_finish_kernel(double *part, int nblocks, double *out)
+{
+    double mx = -INFINITY, bad = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 64) {
+        mx = fmax(mx, part[2 * b]);
+        bad = fmax(bad, part[2 * b + 1]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmax(mx, __shfl_down(mx, o, 64));
+        bad = fmax(bad, __shfl_down(bad, o, 64));
+    }
+    if (threadIdx.x == 0) {
+        out[0] = mx;
+        out[1] = bad;
+    }
+}
+
 template <class S, class T> __global__ void __launch_bounds__(256) inlet_fold_kernel(StepArgs<S, T> a)
 {
     __shared__ double sh[16];
@@ -1351,6 +1409,21 @@ template hipError_t launch_dense_from_plane<float>(const float *, double *, int,
 template hipError_t launch_dense_from_plane<double>(const double *, double *, int, int, int, double, hipStream_t);
 template hipError_t launch_plane_from_dense<float>(const double *, float *, int, int, int, double, hipStream_t);
 template hipError_t launch_plane_from_dense<double>(const double *, double *, int, int, int, double, hipStream_t);
+
+template <class S>
+hipError_t launch_state_scan(const S *src, long long plane, int nx, int ny, int pitch, double *scratch,
+                             int scratch_words, hipStream_t st)
+{
+    // scratch: the 2 result words, then 2 words per block
+    const long long N = (long long)nx * ny;
+    const int nb = (int)std::max(1LL, std::min<long long>((N + 255) / 256, (scratch_words - 2) / 2));
+    if (nb < 1 || 2 * nb + 2 > scratch_words) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((state_scan_kernel<S>), dim3(nb), dim3(256), 0, st, src, plane, nx, ny, pitch, scratch + 2);
+    hipLaunchKernelGGL(scan_finish_kernel, dim3(1), dim3(64), 0, st, scratch + 2, nb, scratch);
+    return hipGetLastError();
+}
+template hipError_t launch_state_scan<float>(const float *, long long, int, int, int, double *, int, hipStream_t);
+template hipError_t launch_state_scan<double>(const double *, long long, int, int, int, double *, int, hipStream_t);
 
 template <class S, class T> hipError_t launch_inlet_fold(const StepArgs<S, T> &a, hipStream_t st)
 {

@@ -57,6 +57,8 @@ program mrlbm_gpu
         nfront = 0; nback = 0; ntop = 0; nbot = 0
     end if
 
+    if (irestart == 1) call check_restart_file()   ! before any device work
+
     call mrlbm_default_params(par)
     par%nx = nx
     par%ny = ny
@@ -190,6 +192,18 @@ program mrlbm_gpu
         rc = mrlbm_step(h, iter, e - iter + 1, opts)
         call check(rc)
         call write_records()
+        ! The reference never checks (NaN propagates into every file silently): at each
+        ! log, plot and restart step the state is scanned on the device, and the run stops
+        ! with exit status 3 before any field file of a non-finite state is written.
+        if (mod(e, 100) == 0 .or. mod(e, max(isave, 1)) == 0 .or. &
+            (post_process .and. mod(e, max(iplot, 1)) == 0) .or. e == max_iter) then
+            rc = mrlbm_check_finite(h)
+            if (rc == MRLBM_E_NONFINITE) then
+                write(*, *) 'STEP: ', e, '     non-finite state (NaN/Inf): stopping'
+                stop 3
+            end if
+            call check(rc)
+        end if
         if (post_process) then
             if (e >= statsbegin .and. e <= statsend .and. mod(e, 100) == 0) then
                 rc = mrlbm_get_rho_inf_mean(h, rho_inf_mean)
@@ -342,11 +356,36 @@ contains
     subroutine read_restart_file(soln_time)      ! 2d_mrlbm.f90:2204-2220
         integer, intent(out) :: soln_time
         real(8), allocatable :: f_eq(:,:,:)
+        integer :: ios
         allocate(f_eq(nx, ny, 0:8))
-        open(unit=10000, file='restart/restart.dat', status="old", form="unformatted")
-            read(10000) soln_time, f, f_eq, ux, uy, rho
+        open(unit=10000, file='restart/restart.dat', status="old", form="unformatted", iostat=ios)
+        if (ios == 0) read(10000, iostat=ios) soln_time, f, f_eq, ux, uy, rho
         close(10000)
         deallocate(f_eq)
+        if (ios /= 0) then
+            write(*, '(a)') "Error while reading restart/restart.dat: truncated, or written for another grid"
+            stop 1
+        end if
+    end subroutine
+
+    ! The reference opens restart.dat and reads one record (2d_mrlbm.f90:2208-2216) with no
+    ! check; a short file then fails inside the Fortran runtime.  Here the file must exist and
+    ! hold at least the record's payload (int32 soln_time and 21 nx*ny arrays of real(8):
+    ! f, f_eq, ux, uy, rho) before the device is touched.
+    subroutine check_restart_file()
+        logical :: there
+        integer(8) :: fsize, need
+        inquire(file='restart/restart.dat', exist=there, size=fsize)
+        need = 4_8 + 8_8*21_8*int(nx, 8)*int(ny, 8)
+        if (.not. there) then
+            write(*, '(a)') "Error: irestart = 1 but restart/restart.dat does not exist"
+            stop 1
+        end if
+        if (fsize >= 0 .and. fsize < need + 8_8) then
+            write(*, '(a,i0,a,i0,a,i0,a,i0)') "Error: restart/restart.dat holds ", fsize, " bytes; the ", nx, &
+                " x ", ny, " record needs ", need + 8_8
+            stop 1
+        end if
     end subroutine
 
     subroutine create_master_p3d_file()      ! 2d_mrlbm.f90:2334-2367

@@ -7,6 +7,7 @@ module mrlbm_iface
     integer(c_int32_t), parameter :: MRLBM_FP64 = 0, MRLBM_FP32_DEV = 1
     integer(c_int32_t), parameter :: MRLBM_MASS_STRONG = 0, MRLBM_MASS_EQ = 1, MRLBM_MASS_WEAK = 2
     integer(c_int32_t), parameter :: MRLBM_INLET_UNIFORM = 0, MRLBM_INLET_PARABOLIC = 1
+    integer(c_int), parameter :: MRLBM_E_NONFINITE = -4
 
     type, bind(C) :: mrlbm_params
         integer(c_int32_t) :: nx, ny
@@ -162,6 +163,10 @@ module mrlbm_iface
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h
             real(c_double), intent(out) :: v
+        end function
+        integer(c_int) function mrlbm_check_finite(h) bind(C, name='mrlbm_check_finite')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
         end function
     end interface
 
