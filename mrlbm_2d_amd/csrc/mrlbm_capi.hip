@@ -639,8 +639,8 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         probe_node[p] = i + h->pitch * j;
     }
     if (opts.post_process && opts.stats_interval > 0 && !h->d_acc) {
-        HIPCHK(hipMalloc(&h->d_acc, (size_t)5 * h->plane * sizeof(double)));
-        HIPCHK(hipMemset(h->d_acc, 0, (size_t)5 * h->plane * sizeof(double)));
+        HIPCHK(hipMalloc(&h->d_acc, (size_t)kAccPlanes * h->plane * sizeof(double)));
+        HIPCHK(hipMemset(h->d_acc, 0, (size_t)kAccPlanes * h->plane * sizeof(double)));
     }
     if (opts.post_process) {
         const int rc = ensure_rec(h);
@@ -1034,8 +1034,8 @@ extern "C" int mrlbm_get_stats(const mrlbm_handle *hc, double *mux, double *muy,
     if (!h->d_acc || h->nsamples == 0) FAIL(MRLBM_E_STATE, "no statistics samples");
     HIPCHK(hipStreamSynchronize(h->st));
     const size_t N = (size_t)h->nx * h->ny;
-    std::vector<double> s[5];
-    for (int q = 0; q < 5; q++) {
+    std::vector<double> s[kAccPlanes];
+    for (int q = 0; q < kAccPlanes; q++) {
         s[q].resize(N);
         const int rc = get_plane<double>(h, h->d_acc + q * h->plane, s[q].data(), 0.0);
         if (rc) FAIL(rc, "device to host copy failed");
@@ -1055,7 +1055,7 @@ extern "C" int mrlbm_reset_stats(mrlbm_handle *h)
 {
     if (!h) return MRLBM_E_ARG;
     h->nsamples = 0;
-    if (h->d_acc) HIPCHK(hipMemsetAsync(h->d_acc, 0, (size_t)5 * h->plane * sizeof(double), h->st));
+    if (h->d_acc) HIPCHK(hipMemsetAsync(h->d_acc, 0, (size_t)kAccPlanes * h->plane * sizeof(double), h->st));
     return 0;
 }
 

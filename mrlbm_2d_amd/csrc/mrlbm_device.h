@@ -247,7 +247,6 @@ __device__ __forceinline__ void node_update(const StepArgs<S, T> &a, int c, int 
         a.acc[a.plane + n] += uy;
         a.acc[2 * a.plane + n] += ux * ux;
         a.acc[3 * a.plane + n] += uy * uy;
-        a.acc[4 * a.plane + n] += ux * uy;
     }
     collide<T>(m, a.om, a.om1, a.incomp_bulk);
 #pragma unroll

@@ -26,6 +26,8 @@ template <class T> inline T *dmalloc(size_t n, hipError_t &e)
     return (T *)p;
 }
 constexpr int kRedWords = 4096;
+// u statistics accumulators (FP64 planes): sum ux, sum uy, sum ux^2, sum uy^2
+constexpr int kAccPlanes = 4;
 
 struct mrlbm_handle {
     mrlbm_params p{};

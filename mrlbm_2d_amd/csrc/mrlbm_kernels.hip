@@ -101,7 +101,6 @@ __device__ __forceinline__ void march_row(const StepArgs<S, T> &a, int c, bool v
                     a.acc[a.plane + n] += uy;
                     a.acc[2 * a.plane + n] += ux * ux;
                     a.acc[3 * a.plane + n] += uy * uy;
-                    a.acc[4 * a.plane + n] += ux * uy;
                 }
             }
             collide<T>(m, a.om, a.om1, a.incomp_bulk);
@@ -237,7 +236,6 @@ __device__ __forceinline__ void march_row_a(const StepArgs<S, T> &a, int c, bool
                 a.acc[a.plane + n] += uy;
                 a.acc[2 * a.plane + n] += ux * ux;
                 a.acc[3 * a.plane + n] += uy * uy;
-                a.acc[4 * a.plane + n] += ux * uy;
             }
         }
         collide<T>(m, a.om, a.om1, a.incomp_bulk);
@@ -340,7 +338,28 @@ template <class S, class T> __device__ __forceinline__ void sample_u(const StepA
     a.acc[a.plane + n] += uy;
     a.acc[2 * a.plane + n] += ux * ux;
     a.acc[3 * a.plane + n] += uy * uy;
-    a.acc[4 * a.plane + n] += ux * uy;
+}
+
+// Both samples of a pair at one node in one read-modify-write of the accumulators: the
+// t+1 velocity (u1, v1) then the t+2 one (u2, v2) added in that order, the rounding of two
+// single-step samples.
+template <class S, class T>
+__device__ __forceinline__ void sample_u_pair(const StepArgs<S, T> &a, long long n, T u1, T v1, T u2, T v2)
+{
+    const double x1 = u1, y1 = v1, x2 = u2, y2 = v2;
+    double s0 = a.acc[n], s1 = a.acc[a.plane + n], s2 = a.acc[2 * a.plane + n], s3 = a.acc[3 * a.plane + n];
+    s0 += x1;
+    s1 += y1;
+    s2 += x1 * x1;
+    s3 += y1 * y1;
+    s0 += x2;
+    s1 += y2;
+    s2 += x2 * x2;
+    s3 += y2 * y2;
+    a.acc[n] = s0;
+    a.acc[a.plane + n] = s1;
+    a.acc[2 * a.plane + n] = s2;
+    a.acc[3 * a.plane + n] = s3;
 }
 
 // Whether sub-step 1 of a two-step pair writes node (c, j) itself: the outer ring, the band
@@ -359,7 +378,9 @@ template <class S, class T> __device__ __forceinline__ bool side_node1(const Ste
 
 // SAMPLE: also accumulate the u statistics: at t+1 (a.sample) of the strip's own rows where
 // sub-step 1 does not write the node, at t+2 (a.sample2) of the nodes the pass writes (with
-// t+1 sampled as well, only those that sub-step 1 does not write).
+// t+1 sampled as well, only those that sub-step 1 does not write).  Where the pass takes
+// both samples of a node, the t+1 velocity is carried one row in registers and both are
+// added in one read-modify-write (sample_u_pair).
 template <class S, class T, int H, bool SAMPLE> __device__ void bulk_march2(const StepArgs<S, T> &a, int bb)
 {
     using F = typename FormOf<T>::F;
@@ -447,6 +468,10 @@ template <class S, class T, int H, bool SAMPLE> __device__ void bulk_march2(cons
     up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
     mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
     T vup_prev[3], vup_cur[3], vmid_cur[3];
+    // t+1 velocity of the node the next level-2 row writes, when both of its samples are
+    // this pass's (SAMPLE only)
+    bool pf = false;
+    T pu = (T)0, pv = (T)0;
 #pragma unroll
     for (int p = 0; p < H + 2; p++) {
         if (p + 3 + LA < NQ) load_moments(a, c, min(j0 - 2 + p + 3 + LA, a.ny - 1), m[p + 3 + LA]);
@@ -456,10 +481,19 @@ template <class S, class T, int H, bool SAMPLE> __device__ void bulk_march2(cons
         T f[Q], m1[6], G9[Q];
         pull_row(left, right, up_prev, mid_cur, F9, in0, in1, in2, f);
         F::moments(f, m1);
+        bool nf = false;
+        T nu_ = (T)0, nv_ = (T)0;
         if constexpr (SAMPLE) {
             const int j1 = j0 - 1 + p;
-            if (a.sample && p >= 1 && p <= H && !side_node1(a, c, j1))
-                sample_u(a, (long long)j1 * a.pitch + c, m1[1], m1[2]);
+            if (a.sample && p >= 1 && p <= H && !side_node1(a, c, j1)) {
+                nf = a.sample2 && two_step_node(a, c, j1); // its t+2 sample comes next row
+                if (nf) {
+                    nu_ = m1[1];
+                    nv_ = m1[2];
+                } else {
+                    sample_u(a, (long long)j1 * a.pitch + c, m1[1], m1[2]);
+                }
+            }
         }
         collide<T>(m1, a.om, a.om1, a.incomp_bulk);
         up_prev[0] = up_cur[0]; up_prev[1] = up_cur[1]; up_prev[2] = up_cur[2];
@@ -477,13 +511,18 @@ template <class S, class T, int H, bool SAMPLE> __device__ void bulk_march2(cons
                 const long long n = (long long)j * a.pitch + c;
                 // when t+1 is sampled too, sub-step 1's side kernel adds t+1 into the nodes
                 // it writes concurrently: those are left to pair_samples_kernel
-                if constexpr (SAMPLE)
-                    if (a.sample2 && !(a.sample && side_node1(a, c, j))) sample_u(a, n, m2[1], m2[2]);
+                if constexpr (SAMPLE) {
+                    if (pf) sample_u_pair(a, n, pu, pv, m2[1], m2[2]);
+                    else if (a.sample2 && !(a.sample && side_node1(a, c, j))) sample_u(a, n, m2[1], m2[2]);
+                }
                 collide<T>(m2, a.om, a.om1, a.incomp_bulk);
 #pragma unroll
                 for (int q = 0; q < 6; q++) a.dst[q * a.plane + n] = (S)m2[q];
             }
         }
+        pf = nf;
+        pu = nu_;
+        pv = nv_;
         if (p >= 1) {
             vup_prev[0] = p == 1 ? vup_prev[0] : vup_cur[0];
             vup_prev[1] = p == 1 ? vup_prev[1] : vup_cur[1];
@@ -750,7 +789,6 @@ template <class S, class T, bool ONE_PASS> __device__ __forceinline__ void wall_
         a.acc[a.plane + n] += vy;
         a.acc[2 * a.plane + n] += vx * vx;
         a.acc[3 * a.plane + n] += vy * vy;
-        a.acc[4 * a.plane + n] += vx * vy;
     }
     collide<T>(m, om, a.om1, a.incomp_bulk);
     S ms[6];
