@@ -41,6 +41,7 @@ program mrlbm_gpu
     integer(8) :: clk0, clk1, clkrate
     real(8) :: wall, mlups, maxux, rho_inf_mean
     character(len=100) :: filename_bin
+    character(len=9) :: plabel
     integer, parameter :: unit_p = 301, unit_f = 302
     integer, parameter :: CHUNK = 256
 
@@ -123,7 +124,8 @@ program mrlbm_gpu
     i_probe(4) = (nfront + ncy + 1) + (2*ncy);   j_probe(4) = (nbot + (ncy/2) + 1)
     open(unit=100, file="data_probe/probe_loc.dat")
         do k = 1, 4
-            write(100, *) "probe ", k, ": ", i_probe(k), j_probe(k)
+            write(plabel, '(a6,i1,a2)') 'probe ', k, ': '   ! one string, as the reference's "probe 1: "
+            write(100, *) plabel, i_probe(k), j_probe(k)
         end do
     close(100)
 

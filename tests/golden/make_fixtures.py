@@ -11,7 +11,9 @@ the reference single-threaded (&Parallel num_threads=1) and stores:
 * the force series data_mean/forces_time.dat (2d_mrlbm.f90:492-494);
 * the last surface-pressure records (2d_mrlbm.f90:609-616);
 * rho_mean_inlet.dat (2d_mrlbm.f90:362-366), theta.dat (2d_mrlbm.f90:1158-1162);
-* r_cyl, nbct, nu, tau from simulation_parameters.dat (2d_mrlbm.f90:157-168).
+* r_cyl, nbct, nu, tau from simulation_parameters.dat (2d_mrlbm.f90:157-168);
+* the size and SHA-256 digest of every file the run wrote (plot3d grid and function
+  files, master.p3d, probe series, ...; files_json).
 
 Usage: python tests/golden/make_fixtures.py   (needs /root/reference and flang)
 """
@@ -191,9 +193,25 @@ def run_case(name, overrides, nsteps, seed):
             rm = os.path.join(d, "rho_mean_inlet.dat")
             if os.path.getsize(rm):
                 out["rho_mean_inlet"] = np.atleast_2d(np.loadtxt(rm))
+        out["files_json"] = np.array(json.dumps(file_digests(d)))
+        out["simulation_parameters"] = np.array(open(os.path.join(d, "simulation_parameters.dat")).read())
         out["params_json"] = np.array(json.dumps(p))
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
         print(f"{name}: {nx}x{ny} steps={nsteps} nbct={out.get('nbct')} ok", flush=True)
+
+
+def file_digests(d):
+    """{relative path: [size, SHA-256]} of every file a run wrote under its directory (the
+    run's input and its restart state included)."""
+    import hashlib
+    out = {}
+    for root, _, names in os.walk(d):
+        for n in names:
+            path = os.path.join(root, n)
+            with open(path, "rb") as fh:
+                data = fh.read()
+            out[os.path.relpath(path, d)] = [len(data), hashlib.sha256(data).hexdigest()]
+    return out
 
 
 STATS_BIN = os.path.join(REPO, "oracle", "_ref", "statistics_ref")
