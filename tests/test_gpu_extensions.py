@@ -112,3 +112,33 @@ def test_parabolic_channel_without_cylinder_matches_oracle(oracle_mod):
         lat.step(2, 2000, opts)
         o.run(2, 2000)
         _check_fields(lat, o)
+
+
+def test_8192x2048_fp64_matches_oracle(oracle_mod):
+    """The benchmark grid itself: 8192 x 2048, D = 128 at (1024, 1025), parabolic inlet
+    u_max = 0.05, Re = 200, from a seeded noisy population state (+-1e-4 velocity), 11 FP64
+    steps (the first one single after the populations are set, then five two-step passes)
+    with the forces of every step: fields bit for bit against the oracle."""
+    from tests.golden import make_fixtures as MF
+    nx, ny, n = 8192, 2048, 11
+    o, p, opts = _pair(oracle_mod, nx, ny, 200.0, 0.05, 1024.0, 1025.0, 64.0, statsbegin=2, statsend=10 ** 7)
+    f0, _, _ = MF.noisy_state(nx, ny, 3)
+    print(f"[gpu] 8192x2048 FP64 x {n} steps vs oracle ({oracle_mod.num_threads()} threads)", flush=True)
+    with capi.Lattice(p) as lat:
+        lat.set_populations(f0)
+        lat.step(2, n, opts)
+        it, gfd, gfl = lat.forces()
+        rho, ux, uy = lat.fields()
+    o.set_state(f0)
+    del f0
+    fd, fl = _run_oracle(o, 2, n, 2)
+    orho, oux, ouy, *_ = o.moments()
+    np.testing.assert_array_equal(rho, orho)
+    np.testing.assert_array_equal(ux, oux)
+    np.testing.assert_array_equal(uy, ouy)
+    np.testing.assert_array_equal(it, np.arange(2, 2 + n, dtype=np.int32))
+    # Early in the run the net force (~1e-4) is a near-cancelling sum of 512 node forces of
+    # size ~0.1: another summation order (a tree on the GPU) moves it by ~1e-15 per node.
+    atol = 1e-15 * o.nbct
+    np.testing.assert_allclose(gfd, fd, rtol=1e-12, atol=atol)
+    np.testing.assert_allclose(gfl, fl, rtol=1e-12, atol=atol)
