@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0
 # steps per launch: FP32 strips of 10 rows, FP64 of 12), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1);
 # the summary and the commands are in profiles/bulk2_kernel_8192x2048_hbm_bytes.md
-TRAFFIC_BYTES = {"f32": 951_367_036, "f64": 2_289_709_604}
+TRAFFIC_BYTES = {"f32": 949_550_578, "f64": 2_289_678_504}
 TRAFFIC_SOURCE = "profiles/bulk2_kernel_8192x2048_hbm_bytes.md"
 NX, NY = 8192, 2048
 # HIP event pairs around every 4th bulk launch on long runs: the per-launch mean without the
