@@ -54,8 +54,25 @@ __device__ __forceinline__ int row_cls(unsigned long long bm, unsigned long long
     return (int)((bm >> t) & 1ull) | ((int)((sm >> t) & 1ull) << 1);
 }
 
-template <class T> __device__ __forceinline__ T shup(T v) { return __shfl_up(v, 1, 64); }
-template <class T> __device__ __forceinline__ T shdn(T v) { return __shfl_down(v, 1, 64); }
+// Whole-wave shifts by one lane, as DPP moves in the VALU (wave_shr:1 / wave_shl:1) rather
+// than LDS permutes: lane i receives lane i-1 (shup) or lane i+1 (shdn); lane 0 (shup) and
+// lane 63 (shdn) receive 0, and the marches never use those lanes' shifted values.
+template <int CTRL> __device__ __forceinline__ int dpp_shift(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <class T, int CTRL> __device__ __forceinline__ T dpp_shift_t(T v)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, dpp_shift<CTRL>(__builtin_bit_cast(int, v)));
+    } else {
+        const long long x = __builtin_bit_cast(long long, v);
+        const int lo = dpp_shift<CTRL>((int)(x & 0xffffffffll)), hi = dpp_shift<CTRL>((int)(x >> 32));
+        return __builtin_bit_cast(T, ((long long)hi << 32) | (long long)(unsigned)lo);
+    }
+}
+template <class T> __device__ __forceinline__ T shup(T v) { return dpp_shift_t<T, 0x138>(v); }
+template <class T> __device__ __forceinline__ T shdn(T v) { return dpp_shift_t<T, 0x130>(v); }
 
 // One row of the bulk march for the wave's 64 columns: rebuild the populations of the
 // row above (moments in mN), assemble the 9 pulled populations of row j, finish the
