@@ -5,7 +5,9 @@ lift spectrum's St and the final velocity field stated beside it.
 * 1600 x 400 channel, D = 40 at (200, 201), parabolic inlet u_max = 0.06, Re = 100,
   seed 2, 200,000 steps, statistics over the last 100,000 (vortex shedding); also two
   steps per bulk pass against single steps over 2,000 recorded steps from the state
-  reached, and St by both the reference's period rule and the spectrum;
+  reached, and St by both the reference's period rule and the spectrum; and the same
+  FP64 run's mean / RMS velocity fields (sampled every 10th step) checked for their
+  sample count, the imposed inlet and wall values and the RMS peak in the wake;
 * 8192 x 2048, D = 128 at (1024, 1025), u_max = 0.05, Re = 200, seed 3: a bounded
   20,000 steps with Cd over the last 10,000 (the full 100,000-step comparison runs in
   the benchmark).
@@ -74,3 +76,31 @@ def test_fp32_gate_8192x2048_bounded():
     print("[gpu] precision gate 8192x2048 Re=200, 20000 steps, window 10000", flush=True)
     res = compare.compare(8192, 2048, 1024.0, 1025.0, 64.0, 200.0, 0.05, 20000, 10000, seed=3)
     _check_gate(res)
+
+
+def test_u_statistics_of_shedding_run_1600x400():
+    """The 1600 x 400 configuration as stated: 200,000 FP64 steps with mean / RMS velocity
+    fields over the last 100,000 (a sample every 10th step): the samples are counted right,
+    the inlet column's mean is the imposed parabola with zero fluctuation, the walls' mean is
+    zero, and the fluctuations peak in the cylinder's wake (shedding) and not upstream."""
+    p = compare.make_params(1600, 400, 200.0, 201.0, 20.0, 100.0, 0.06, capi.MRLBM_FP64)
+    with capi.Lattice(p) as lat:
+        rng = np.random.default_rng(2)
+        lat.set_moments(np.ones((1600, 400)), rng.uniform(-1e-4, 1e-4, (1600, 400)),
+                        rng.uniform(-1e-4, 1e-4, (1600, 400)))
+        print("[gpu] 1600x400 FP64 200000 steps, u statistics over the last 100000", flush=True)
+        lat.step(2, 100000)
+        o = capi.default_step_opts()
+        o.post_process, o.statsbegin, o.statsend, o.stats_interval = 1, 100002, 200001, 10
+        lat.step(100002, 100000, o)
+        mux, muy, rux, ruy, ns = lat.stats()
+        assert lat.check_finite() == 0
+    assert ns == 10000
+    s = np.arange(400) / 399.0
+    np.testing.assert_allclose(mux[0], 4.0 * 0.06 * s * (1.0 - s), rtol=1e-12, atol=1e-17)
+    assert np.abs(muy[0]).max() == 0.0 and rux[0].max() < 1e-7 and ruy[0].max() < 1e-7
+    assert np.abs(mux[:, 0]).max() == 0.0 and np.abs(mux[:, -1]).max() == 0.0
+    wake = ruy[320:600, 150:252].max()      # 3 to 10 diameters behind the cylinder's centre
+    upstream = ruy[20:100, 150:252].max()   # 2 to 4.5 diameters ahead of it
+    print(f"[gpu] rms uy: wake {wake:.3e}, upstream {upstream:.3e}", flush=True)
+    assert wake > 1e-3 and wake > 5 * upstream
