@@ -65,7 +65,12 @@ typedef struct mrlbm_params {
                                      over the plain interior (strips of that many rows), the edges
                                      and the cylinder band in two ordinary sub-steps; -1 (default)
                                      auto: 10 rows FP32, 12 rows FP64.  Same bits either way */
-    int32_t reserved[4];
+    int32_t steps_per_pass;       /* with time_blocking on: 0 (default) or 4: four steps per bulk pass
+                                     over the plain interior where four consecutive steps allow it
+                                     (grids of at least 32 x 32; no u-statistics sample, probe record
+                                     or plot step among them), else pairs; 2: pairs only.  Same bits
+                                     either way */
+    int32_t reserved[3];
 } mrlbm_params;
 
 /* What one call to mrlbm_step records (2d_mrlbm.f90:355-502). */
@@ -265,12 +270,12 @@ int mrlbm_batch_get_info(const mrlbm_batch *b, mrlbm_batch_info *info);
 int mrlbm_set_timing(mrlbm_handle *h, int32_t on);
 int mrlbm_last_step_ms(const mrlbm_handle *h, double *ms);
 /* Mean duration (ms) of one bulk-kernel launch over the last timed mrlbm_step call (a
- * two-step launch covers two iterations), from the start/end event pairs of the timed
+ * two- or four-step launch covers two or four iterations), from the start/end event pairs of the timed
  * launches (at most 4096); *launches = number of launches timed.
  * New: no counterpart in the reference (its loop prints MLUPS only, 2d_mrlbm.f90:540-553). */
 int mrlbm_last_bulk_ms(const mrlbm_handle *h, double *ms_per_launch, int32_t *launches);
 /* Bulk launches of the last mrlbm_step call: all of them, and those timed in timing mode.
- * Two-step launches advance 2 steps, single-step launches 1; timed_steps counts the steps
+ * Four-step launches advance 4 steps, two-step launches 2, single-step launches 1; timed_steps counts the steps
  * the timed launches advanced, so timed_steps * bytes per step / timed_ms is exact whatever
  * mix of launches ran.  New: no counterpart in the reference. */
 typedef struct mrlbm_timing {

@@ -49,7 +49,7 @@ class Params(C.Structure):
         ("incomp_bc", C.c_int32), ("incomp_bulk", C.c_int32),
         ("precision", C.c_int32), ("label_policy", C.c_int32), ("strip_rows", C.c_int32),
         ("kernel", C.c_int32), ("block_order", C.c_int32), ("time_blocking", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("steps_per_pass", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 

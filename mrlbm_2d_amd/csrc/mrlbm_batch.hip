@@ -461,8 +461,10 @@ extern "C" int mrlbm_batch_create(const mrlbm_params *runs, int32_t nruns, const
         mrlbm_handle *h = nullptr;
         mrlbm_params pr = runs[r];
         // any strip height: it allocates the scratch state and the band rectangles of the
-        // two-step pass (the batch picks the strip height itself)
+        // two-step pass (the batch picks the strip height itself); the batch steps in pairs, so
+        // no four-step scratch state
         pr.time_blocking = opts.time_blocking == 0 ? 0 : 2;
+        pr.steps_per_pass = 2;
         const int rc = mrlbm_create(&pr, &h);
         if (rc) return fail(rc, "run " + std::to_string(r) + ": " + mrlbm_last_error(nullptr));
         b->runs.push_back(h);

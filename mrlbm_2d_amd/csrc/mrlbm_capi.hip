@@ -224,7 +224,7 @@ static void free_all(mrlbm_handle *h)
 {
     adjoint_free(h);
     for (void *p : {h->S[0], h->S[1], h->pops, h->ghost, h->uin, (void *)h->d_boxbits, (void *)h->d_rec, (void *)h->d_inlet[0],
-                    (void *)h->d_inlet[1], (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red, h->chi, h->S2, (void *)h->d_dense})
+                    (void *)h->d_inlet[1], (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red, h->chi, h->S2, h->S3, (void *)h->d_dense})
         if (p) (void)hipFree(p);
     for (void *p : h->allocs)
         if (p) (void)hipFree(p);
@@ -294,6 +294,34 @@ int mrlbm::set_bulk_shape(mrlbm_handle *h, int kernel_in, int strip_rows, std::s
     return 0;
 }
 
+// Band rectangles of a sub-step that writes the L layers inside the outer ring and the cylinder
+// box dilated by L, the latter grown to hold the wall's stencil nodes dilated by G (G = 0: not
+// grown).  A sub-step's wall kernels read the previous sub-step's state on the stencil nodes
+// and their neighbours, and the plain updates there read one layer further out in the
+// sub-step before: with n sub-steps, sub-step s holds the stencil nodes dilated by n - s.
+// Returns how many rectangles.
+static int band_rects(const mrlbm_handle *h, int L, int G, int r[5][4])
+{
+    const int nx = h->nx, ny = h->ny;
+    auto put = [](int q[4], int x0, int y0, int x1, int y1) { q[0] = x0; q[1] = y0; q[2] = x1 - x0 + 1; q[3] = y1 - y0 + 1; };
+    put(r[0], 1, 1, nx - 2, L);
+    put(r[1], 1, ny - 1 - L, nx - 2, ny - 2);
+    put(r[2], 1, L + 1, L, ny - 2 - L);
+    put(r[3], nx - 1 - L, L + 1, nx - 2, ny - 2 - L);
+    if (h->bt.bw <= 0) return 4;
+    const BoundaryTables &t = h->bt;
+    int x0 = t.bx0 - L, x1 = t.bx0 + t.bw - 1 + L, y0 = t.by0 - L, y1 = t.by0 + t.bh - 1 + L;
+    if (G > 0)
+        for (int u = 0; u < t.nU; u++) {
+            x0 = std::min(x0, t.uij[2 * u] - G);
+            x1 = std::max(x1, t.uij[2 * u] + G);
+            y0 = std::min(y0, t.uij[2 * u + 1] - G);
+            y1 = std::max(y1, t.uij[2 * u + 1] + G);
+        }
+    put(r[4], std::max(x0, 1), std::max(y0, 1), std::min(x1, nx - 2), std::min(y1, ny - 2));
+    return 5;
+}
+
 // Two-step pass: the scratch state and the band rectangles.  Sub-step 1 (t -> t+1 into the
 // scratch) covers the two layers inside the ring and the cylinder box grown to hold every
 // node the wall kernels and sub-step 2 read; sub-step 2 (t+1 -> t+2) covers the layer
@@ -307,30 +335,26 @@ static int setup_time_blocking(mrlbm_handle *h, int H)
     h->nstrips2 = (ny - 4 + H - 1) / H; // output rows 2..ny-3
     h->ncg2 = (nx + 63) / 64;            // aligned 64-column waves whatever the single-step variant
     h->ncb2 = (h->ncg2 + 3) / 4;
-    auto put = [](int r[4], int x0, int y0, int x1, int y1) { r[0] = x0; r[1] = y0; r[2] = x1 - x0 + 1; r[3] = y1 - y0 + 1; };
-    put(h->rect1[0], 1, 1, nx - 2, 2);
-    put(h->rect1[1], 1, ny - 3, nx - 2, ny - 2);
-    put(h->rect1[2], 1, 3, 2, ny - 4);
-    put(h->rect1[3], nx - 3, 3, nx - 2, ny - 4);
-    put(h->rect2[0], 1, 1, nx - 2, 1);
-    put(h->rect2[1], 1, ny - 2, nx - 2, ny - 2);
-    put(h->rect2[2], 1, 2, 1, ny - 3);
-    put(h->rect2[3], nx - 2, 2, nx - 2, ny - 3);
-    h->nrect1 = h->nrect2 = 4;
-    if (h->bt.bw > 0) {
-        const BoundaryTables &t = h->bt;
-        int x0 = t.bx0 - 2, x1 = t.bx0 + t.bw + 1, y0 = t.by0 - 2, y1 = t.by0 + t.bh + 1;
-        for (int u = 0; u < t.nU; u++) {
-            x0 = std::min(x0, t.uij[2 * u] - 1);
-            x1 = std::max(x1, t.uij[2 * u] + 1);
-            y0 = std::min(y0, t.uij[2 * u + 1] - 1);
-            y1 = std::max(y1, t.uij[2 * u + 1] + 1);
-        }
-        put(h->rect1[4], std::max(x0, 1), std::max(y0, 1), std::min(x1, nx - 2), std::min(y1, ny - 2));
-        put(h->rect2[4], std::max(t.bx0 - 1, 1), std::max(t.by0 - 1, 1), std::min(t.bx0 + t.bw, nx - 2),
-            std::min(t.by0 + t.bh, ny - 2));
-        h->nrect1 = h->nrect2 = 5;
-    }
+    h->nrect1 = band_rects(h, 2, 1, h->rect1);
+    h->nrect2 = band_rects(h, 1, 0, h->rect2);
+    return 0;
+}
+
+// Four-step pass: sub-step s = 1..4 writes state t+s on the 7-s layers inside the ring and
+// the cylinder box dilated by 7-s (grown to the stencil nodes dilated by 4-s), into S2, S3, S2
+// and the destination; the four-step bulk writes the nodes four or more from the ring and
+// the box.  Strips of 24 rows (FP32) or 16 (FP64), halved when that leaves fewer than 4096
+// waves.
+static int setup_four_step(mrlbm_handle *h)
+{
+    const int nx = h->nx, ny = h->ny;
+    HIPCHK(hipMalloc(&h->S3, (size_t)6 * h->plane * (h->f32 ? sizeof(float) : sizeof(double))));
+    const int wide = h->f32 ? 24 : 16;
+    h->ncgN = (nx - 8 + 55) / 56; // output columns 4..nx-5, 56 per wave
+    h->ncbN = (h->ncgN + 3) / 4;
+    h->ms_H = (long long)h->ncgN * ((ny - 8 + wide - 1) / wide) >= 4096 ? wide : wide / 2;
+    h->nstripsN = (ny - 8 + h->ms_H - 1) / h->ms_H; // output rows 4..ny-5
+    for (int s = 1; s <= 4; s++) h->nrectq[s - 1] = band_rects(h, 7 - s, 4 - s, h->rectq[s - 1]);
     return 0;
 }
 
@@ -415,9 +439,15 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
         // -1 auto: the strip height measured fastest at 8192 x 2048 on MI355X
         const int tb = p->time_blocking < 0 ? (h->f32 ? 10 : 12) : p->time_blocking;
         if (tb < 0 || tb > 16 || tb % 2) return fail(MRLBM_E_ARG, "time_blocking must be -1 or an even number 0..16");
+        if (p->steps_per_pass != 0 && p->steps_per_pass != 2 && p->steps_per_pass != 4)
+            return fail(MRLBM_E_ARG, "steps_per_pass must be 0, 2 or 4");
         if (tb && h->nx >= 16 && h->ny >= 16) {
             const int rc3 = setup_time_blocking(h, tb);
             if (rc3) return fail(rc3, "");
+        }
+        if (tb && p->steps_per_pass != 2 && h->nx >= 32 && h->ny >= 32) {
+            const int rc4 = setup_four_step(h);
+            if (rc4) return fail(rc4, "");
         }
     }
     *out = h;
@@ -714,6 +744,21 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         const int nrec = (is_window(iter) ? 1 : 0) + (is_window(iter + 1) ? 1 : 0);
         return (int)rec_iters.size() + nrec <= std::max(h->rec_cap, 0) || nrec == 0;
     };
+    // four iterations may share one four-step bulk pass when none of them samples the u
+    // statistics, records probes (the bulk carries neither) or zeroes the solid nodes' velocity,
+    // and the records fit
+    auto quad_ok = [&](int iter) {
+        if (!h->ms_H || !h->tb_H || h->from_pops || h->chi || h->p.kernel == 1) return false;
+        int nrec = 0;
+        for (int it = iter; it <= iter + 3; it++) {
+            if (plot_step(it) || sample_step(it)) return false;
+            if (is_window(it)) {
+                if (opts.nprobe > 0) return false;
+                nrec++;
+            }
+        }
+        return (int)rec_iters.size() + nrec <= std::max(h->rec_cap, 0) || nrec == 0;
+    };
     // timing mode: the start and end times of every h->timing-th bulk launch (the first
     // included) into an event pair, at most kMaxTimed launches
     constexpr int kMaxTimed = 4096;
@@ -721,7 +766,7 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
     h->last_ms = h->last_bulk_ms = 0.0;
     h->last_bulk_n = 0;
     h->tstat = mrlbm_timing{};
-    std::vector<int> timed_steps; // steps advanced by each timed launch (1 or 2)
+    std::vector<int> timed_steps; // steps advanced by each timed launch (1, 2 or 4)
     if (h->timing) {
         const size_t want = 2 * (size_t)std::min(nsteps / h->timing + 1, kMaxTimed);
         while (h->tev.size() < want) {
@@ -754,6 +799,13 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         h->bulk_done = t1;
         return launch_bulk2<S, T>(bulk, h->ncb2 * h->nstrips2, h->tb_H, h->st, t0, t1);
     };
+    auto launch_bulk_quad = [&](const StepArgs<S, T> &bulk) -> hipError_t {
+        hipEvent_t t0, t1;
+        timed_pair(t0, t1, 4);
+        if (!t1) t1 = h->evK;
+        h->bulk_done = t1;
+        return launch_bulkn<S, T>(bulk, h->ncbN * h->nstripsN, h->ms_H, h->st, t0, t1);
+    };
     auto launch_bulk_one = [&](const StepArgs<S, T> &a, int nb) -> hipError_t {
         hipEvent_t t0, t1;
         timed_pair(t0, t1, 1);
@@ -763,6 +815,33 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
     };
     for (int s = 0; s < nsteps; s++) {
         const int iter = iter_first + s;
+        if (s + 3 < nsteps && quad_ok(iter)) {
+            // the bulk: state t -> t+4 on the interior; stream B: sub-steps 1..4 through the
+            // scratch states, each the ring, bands, stencil and wall work of its iteration
+            void *A = h->S[h->cur], *C = h->S[h->cur ^ 1];
+            void *src[4] = {A, h->S2, h->S3, h->S2}, *dst[4] = {h->S2, h->S3, h->S2, C};
+            auto bulk = step_args(iter, A, C, -1);
+            bulk.ncg = h->ncgN;
+            bulk.ncb = h->ncbN;
+            bulk.nstrips = h->nstripsN;
+            HIPCHK(hipStreamWaitEvent(h->st2, h->bulk_done, 0));
+            HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
+            HIPCHK(launch_bulk_quad(bulk));
+            int slot = (int)rec_iters.size();
+            for (int k = 0; k < 4; k++) {
+                auto a = step_args(iter + k, src[k], dst[k], slot);
+                if (is_window(iter + k)) slot++;
+                a.nrect = h->nrectq[k];
+                std::memcpy(a.rect, h->rectq[k], sizeof h->rectq[k]);
+                HIPCHK(side_work(a));
+                after_iter(iter + k);
+            }
+            HIPCHK(hipEventRecord(h->evB, h->st2));
+            h->cur ^= 1;
+            for (int k = 0; k < 4; k++) record_iter(iter + k);
+            s += 3;
+            continue;
+        }
         if (s + 1 < nsteps && pair_ok(iter)) {
             void *A = h->S[h->cur], *C = h->S[h->cur ^ 1];
             const int slot1 = (int)rec_iters.size();

@@ -90,6 +90,12 @@ struct mrlbm_handle {
     double *d_dense = nullptr;              // nx*ny FP64 staging for the host copies (on first use)
     int rect1[5][4] = {}, rect2[5][4] = {}; // band rectangles of sub-steps 1 and 2
     int nrect1 = 0, nrect2 = 0;
+    // four-step pass: a second scratch state (sub-steps 1..3 alternate between S2 and S3), the
+    // strip height, strips, waves (56 written columns each), blocks per strip, and the band
+    // rectangles of sub-steps 1..4
+    void *S3 = nullptr;
+    int ms_H = 0, nstripsN = 0, ncgN = 0, ncbN = 0;
+    int rectq[4][5][4] = {}, nrectq[4] = {};
     AdjointWork *adj = nullptr; // buffers of the discrete adjoint, allocated on first use
 };
 

@@ -110,6 +110,12 @@ hipError_t launch_stencil_moments(const StepArgs<S, T> &a, hipStream_t st);
 template <class S, class T>
 hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0 = nullptr,
                         hipEvent_t t1 = nullptr);
+// Four steps in one pass over the plain interior (nodes four or more away from the edges and
+// from the cylinder box), src = state t, dst = state t+4; waves of 64 columns writing the
+// middle 56, strips of H rows (FP32 12 or 24, FP64 8 or 16).
+template <class S, class T>
+hipError_t launch_bulkn(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0 = nullptr,
+                        hipEvent_t t1 = nullptr);
 // Workgroups of the ring's inlet and outlet columns when they go by column tiles (64 rows
 // each, staged in LDS): grids at least 8 wide stepping from moments; 0 otherwise (the
 // columns are then ordinary ring items).

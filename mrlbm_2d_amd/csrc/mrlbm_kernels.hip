@@ -558,6 +558,101 @@ __global__ void __launch_bounds__(256) bulk2_kernel(StepArgs<S, T> a)
     bulk_march2<S, T, H, SAMPLE>(a, blockIdx.x);
 }
 
+// Output nodes of an NS-step pass: NS or more nodes from the outer ring and outside the
+// cylinder box dilated by NS-1 (everything else is finished by the band sub-steps).
+template <class S, class T, int NS> __device__ __forceinline__ bool n_step_node(const StepArgs<S, T> &a, int c, int j)
+{
+    if (c < NS || c > a.nx - 1 - NS || j < NS || j > a.ny - 1 - NS) return false;
+    return !(a.bw > 0 && c >= a.bx0 - (NS - 1) && c <= a.bx0 + a.bw + (NS - 2) && j >= a.by0 - (NS - 1) &&
+             j <= a.by0 + a.bh + (NS - 2));
+}
+
+// NS steps in one pass over the plain interior, src = state t, dst = state t+NS.  A wave holds
+// 64 consecutive columns and writes the middle 64 - 2 NS of them: state t+s computed in all
+// lanes is right in lanes s .. 63-s (each level loses one column per side to the lanes whose
+// x-shifted inputs come from outside the wave), so waves share no data and overlap by 2 NS
+// columns instead.  The wave marches up a strip of H output rows: per row of state t loaded
+// (LA rows ahead), level s = 1 .. NS pulls its next row from the three rows of level s-1 it
+// carries (k = 2, 5, 6 of the row below, 0, 1, 3 of the row itself, 4, 7, 8 of the row just
+// made above) and collides it; levels below NS rebuild the row's populations for the level
+// above, level NS stores its moments.  Level s makes the rows j0-NS+s .. j0+H+NS-1-s, the
+// first of them at march step 2s.  Per node and level the arithmetic is the single step's, so
+// the result is bit-identical to NS single steps.
+template <class S, class T, int NS, int H> __device__ void bulk_march_n(const StepArgs<S, T> &a, int bb)
+{
+    using F = typename FormOf<T>::F;
+    constexpr int W = 64 - 2 * NS, NP = H + 2 * NS, LA = 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = (bb % a.ncb) * 4 + wave, st = bb / a.ncb;
+    if (g >= a.ncg) return;
+    const int c = g * W + lane, cl = min(c, a.nx - 1); // lanes past the last column load it again
+    const bool outc = lane >= NS && lane < 64 - NS;
+    const int j0 = NS + st * H; // output rows j0 .. j0+H-1; state t rows j0-NS .. j0+H+NS-1
+    const S *__restrict__ src = a.src; // the pass writes another state than it reads
+    S *__restrict__ dst = a.dst;
+    const long long plane = a.plane;
+    auto load_row = [&](int p, S m[6]) {
+        const long long n = (long long)min(j0 - NS + p, a.ny - 1) * a.pitch + cl;
+#pragma unroll
+        for (int k = 0; k < 6; k++) m[k] = src[k * plane + n];
+    };
+    S q[LA + 1][6];
+#pragma unroll
+    for (int i = 0; i <= LA; i++) load_row(i, q[i]);
+    T upp[NS + 1][3], upc[NS + 1][3], mid[NS + 1][3];
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        T Fin[Q]; // populations of the newest row of the level below
+        F::recon(q[0], Fin);
+#pragma unroll
+        for (int i = 0; i < LA; i++)
+#pragma unroll
+            for (int k = 0; k < 6; k++) q[i][k] = q[i + 1][k];
+        if (p + LA + 1 < NP) load_row(p + LA + 1, q[LA]);
+#pragma unroll
+        for (int s = 1; s <= NS; s++) {
+            if (p < 2 * s - 2) break; // level s starts two march steps after level s-1
+            T f[Q];
+            const bool make = p >= 2 * s;
+            if (make) {
+                f[0] = mid[s][0];
+                f[1] = shup(mid[s][1]);
+                f[3] = shdn(mid[s][2]);
+                f[2] = upp[s][0];
+                f[5] = shup(upp[s][1]);
+                f[6] = shdn(upp[s][2]);
+                f[4] = Fin[4];
+                f[7] = shdn(Fin[7]);
+                f[8] = shup(Fin[8]);
+            }
+            upp[s][0] = upc[s][0]; upp[s][1] = upc[s][1]; upp[s][2] = upc[s][2];
+            upc[s][0] = Fin[2]; upc[s][1] = Fin[5]; upc[s][2] = Fin[6];
+            mid[s][0] = Fin[0]; mid[s][1] = Fin[1]; mid[s][2] = Fin[3];
+            if (!make) break; // its window fills at steps 2s-2 and 2s-1
+            T m[6];
+            F::moments(f, m);
+            collide<T>(m, a.om, a.om1, a.incomp_bulk);
+            if (s < NS) {
+                F::recon(m, Fin);
+            } else {
+                const int j = j0 + p - 2 * NS;
+                if (outc && n_step_node<S, T, NS>(a, c, j)) {
+                    const long long n = (long long)j * a.pitch + c;
+#pragma unroll
+                    for (int k = 0; k < 6; k++) dst[k * plane + n] = (S)m[k];
+                }
+            }
+        }
+    }
+}
+
+// at least two waves per SIMD (left alone, the FP64 kernel takes 256 VGPRs plus AGPRs: one)
+template <class S, class T, int NS, int H>
+__global__ void __launch_bounds__(256, 2) bulkn_kernel(StepArgs<S, T> a)
+{
+    bulk_march_n<S, T, NS, H>(a, blockIdx.x);
+}
+
 // The side stream's node work of one step in one launch: the outer ring (edge update),
 // the band rectangles of a two-step pair (plain update) and the curved wall's stencil
 // nodes (pre-collision moments for boundary_kernel).  All three read only the source
@@ -1391,6 +1486,22 @@ hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t
 #undef MRLBM_BULK2
     return hipGetLastError();
 }
+template <class S, class T>
+hipError_t launch_bulkn(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0, hipEvent_t t1)
+{
+    const dim3 g(nblocks), b(256);
+    // strip heights: FP32 24 (12 on small grids), FP64 16 (8)
+    if constexpr (sizeof(S) == 4) {
+        if (H == 24) launch_timed(bulkn_kernel<S, T, 4, 24>, g, b, st, t0, t1, a);
+        else if (H == 12) launch_timed(bulkn_kernel<S, T, 4, 12>, g, b, st, t0, t1, a);
+        else return hipErrorInvalidValue;
+    } else {
+        if (H == 16) launch_timed(bulkn_kernel<S, T, 4, 16>, g, b, st, t0, t1, a);
+        else if (H == 8) launch_timed(bulkn_kernel<S, T, 4, 8>, g, b, st, t0, t1, a);
+        else return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st, bool stencil)
 {
     const int nring = 2 * a.nx + 2 * (a.ny - 2);
@@ -1513,6 +1624,7 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_edges<S, T>(const StepArgs<S, T> &, hipStream_t);                 \
     template hipError_t launch_stencil_moments<S, T>(const StepArgs<S, T> &, hipStream_t);       \
     template hipError_t launch_bulk2<S, T>(const StepArgs<S, T> &, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
+    template hipError_t launch_bulkn<S, T>(const StepArgs<S, T> &, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
     template hipError_t launch_side<S, T>(const StepArgs<S, T> &, hipStream_t, bool);            \
     template hipError_t launch_wall<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
     template hipError_t launch_bulk_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, int, hipStream_t); \

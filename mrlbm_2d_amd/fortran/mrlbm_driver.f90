@@ -25,6 +25,7 @@ program mrlbm_gpu
     logical :: vel_interp, mom_interp, rotated_coordinate, post_process
     logical :: strong_mass_bc, eq_mass_bc, weak_mass_bc
     integer :: precision = 0, inlet_profile = 0, strip_rows = 0, kernel = 0, time_blocking = -1
+    integer :: steps_per_pass = 0
     logical :: record_surface_p = .true.
 
     integer :: i_probe(4), j_probe(4)
@@ -85,6 +86,7 @@ program mrlbm_gpu
     par%strip_rows = strip_rows
     par%kernel = kernel
     par%time_blocking = time_blocking
+    par%steps_per_pass = steps_per_pass
     rc = mrlbm_create(par, h)
     if (rc /= 0) then
         write(*, *) 'mrlbm_create failed: ', trim(mrlbm_error_string(c_null_ptr))
@@ -425,7 +427,8 @@ contains
         namelist/Controls/uo, rho_infty, iplot, max_iter, isave, irestart, statsbegin, statsend, iplotstats
         namelist/LogicalControls/post_process, x_periodic, y_periodic, channel_with_cylinder, channel_with_square, &
             & incomp_bc, incomp_bulk, vel_interp, mom_interp, rotated_coordinate, strong_mass_bc, eq_mass_bc, weak_mass_bc
-        namelist/Accelerator/precision, inlet_profile, strip_rows, kernel, record_surface_p, time_blocking
+        namelist/Accelerator/precision, inlet_profile, strip_rows, kernel, record_surface_p, time_blocking, &
+            & steps_per_pass
 
         open(newunit=u, file="input.dat", status="old", iostat=ios)
         if (ios /= 0) then

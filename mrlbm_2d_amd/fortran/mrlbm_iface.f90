@@ -19,8 +19,8 @@ module mrlbm_iface
         integer(c_int32_t) :: vel_interp, mom_interp
         integer(c_int32_t) :: incomp_bc, incomp_bulk
         integer(c_int32_t) :: precision, label_policy, strip_rows, kernel, block_order
-        integer(c_int32_t) :: time_blocking
-        integer(c_int32_t) :: reserved(4)
+        integer(c_int32_t) :: time_blocking, steps_per_pass
+        integer(c_int32_t) :: reserved(3)
     end type
 
     type, bind(C) :: mrlbm_step_opts

@@ -7,7 +7,7 @@ leave a handle as they found it.
   maximum of the fields it returns.
 * The driver stops with exit status 3 at the first log step of a non-finite state (a NaN
   population in restart.dat), before writing a field or restart file of that state.
-* mrlbm_adjoint_drag leaves the running inlet-density mean and the two-step path as they
+* mrlbm_adjoint_drag leaves the running inlet-density mean and the multi-step passes as they
   were (it installs a zero Brinkman field for its sweep and removes it again).
 """
 import os
@@ -105,6 +105,6 @@ def test_adjoint_leaves_the_handle_as_it_was(precision):
         lat.set_timing(1)
         lat.step(702, 20)
         t = lat.last_timing()
-        # two steps per bulk pass again (a Brinkman field would force single steps)
-        assert t.bulk_launches == 10 and t.bulk_steps == 20
+        # four steps per bulk pass again (a Brinkman field would force single steps)
+        assert t.bulk_launches == 5 and t.bulk_steps == 20
         assert lat.check_finite() == 0

@@ -125,21 +125,27 @@ def test_fp32_variants_identical_and_close_to_reference(name):
     assert duy < max(5e-5 * umax / 0.05, 5e-7)
 
 
-@pytest.mark.parametrize("time_blocking,every,launches", [(0, 1, 10), (4, 1, 5), (0, 4, 3), (4, 4, 2)])
-def test_bulk_timing_counts_launches(time_blocking, every, launches):
+@pytest.mark.parametrize("time_blocking,steps_per_pass,every,launches,steps",
+                         [(0, 0, 1, 10, 10), (4, 2, 1, 5, 10), (0, 0, 4, 3, 3), (4, 2, 4, 2, 4), (4, 0, 1, 3, 10),
+                          (4, 0, 2, 2, 6)])
+def test_bulk_timing_counts_launches(time_blocking, steps_per_pass, every, launches, steps):
     """Timing mode records an event pair around every n-th bulk launch (a two-step launch
-    covers two steps; 10 steps = 10 or 5 launches) and leaves the result unchanged."""
+    covers two steps, a four-step launch four: 10 steps = 10, 5 or 3 launches, the last of
+    them a pair) and leaves the result unchanged; the timed launches' steps are counted."""
     g = gc.load("p1_walls")
     out = []
     for timing in (0, every):
         par = params_from_namelist(g["params"])
         par.time_blocking = time_blocking
+        par.steps_per_pass = steps_per_pass
         with Lattice(par) as lat:
-            print(f"[gpu] timing={timing} time_blocking={time_blocking}", flush=True)
+            print(f"[gpu] timing={timing} time_blocking={time_blocking} steps_per_pass={steps_per_pass}", flush=True)
             lat.set_timing(timing)
             lat.step(2, 10)
             ms, n = lat.last_bulk_ms()
             if timing:
+                t = lat.last_timing()
+                assert t.timed_steps == steps and t.bulk_steps == 10
                 assert n == launches
                 assert 0 < ms < 1e3
                 assert lat.last_step_ms() > 0
@@ -154,7 +160,7 @@ def test_large_grid_matches_oracle(oracle_mod):
     """A grid of 4.7M nodes with the cylinder wall 64 nodes wide.  FP64 at 4352 x 1088 (the reference's
     L_front/back/top/bot = 8/55/8/7 with ncy = 68) against the oracle, from the rest state
     with seeded +-1e-3 relative noise on every population: a single step from populations,
-    two two-step passes, a single step; fields bit for bit and the drag/lift series."""
+    one four-step pass, a single step; fields bit for bit and the drag/lift series."""
     p = dict(gc.load("p1_walls")["params"])
     p.update(ncy=68, L_front=8, L_back=55, L_top=8, L_bot=7, Re=200.0, uo=0.05, nx=0, ny=0,
              iplot=100000, statsbegin=2)
