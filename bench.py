@@ -4,7 +4,7 @@ Workload (the perf configuration of the metric): 8192 x 2048 channel, cylinder D
 centred at (1024, 1025), parabolic inlet u_max = 0.05, Re = 200, seeded +-1e-4 initial
 velocity noise.  Both precision forms run: FP32 deviation storage (the headline `value`)
 and plain FP64.  A "step" is one lattice update of every node; the default path advances
-two steps per launch of the bulk kernel (temporal blocking), bit-identical to single steps.
+four steps per launch of the bulk kernel (temporal blocking), bit-identical to single steps.
 
 Prints ONE JSON line.  `roofline` prices the dominant kernel against MI355X's 8 TB/s HBM:
 achieved = algorithmic bytes of the timed launches (12 moment words per node and step: 6
@@ -13,7 +13,9 @@ summed duration, from a HIP event pair given to each timed launch on the bulk ke
 stream (every launch when the run is short, every 4th otherwise);
 traffic = one launch's HBM bytes from rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE) of the
 committed profile named in `traffic_source`, not measured in this run; below the algorithmic
-bytes because the intermediate step never leaves the chip.
+bytes because the intermediate steps never leave the chip, so `frac` (algorithmic) passes 1
+and `traffic_frac` (those bytes over the same launch time, against 8 TB/s) is the share of HBM
+bandwidth the launch actually uses.
 `fp32_vs_fp64` is the accuracy condition of the FP32 figure: both forms from the same seeded
 state over 100,000 steps (mean Cd, rms Cl, spectral St over the last 50,000; velocity L2 at
 the end), run before the timed legs.  Also: the 512x128 drag adjoint (adjoint / forward
@@ -37,12 +39,15 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the dominant kernel at 8192 x 2048 (the two-step bulk pass, two
-# steps per launch: FP32 strips of 10 rows, FP64 of 12), from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1);
-# the summary and the commands are in profiles/bulk2_kernel_8192x2048_hbm_bytes.md
-TRAFFIC_BYTES = {"f32": 949_550_578, "f64": 2_289_678_504}
-TRAFFIC_SOURCE = "profiles/bulk2_kernel_8192x2048_hbm_bytes.md"
+# HBM bytes per launch of the dominant kernel at 8192 x 2048, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1):
+# the four-step bulk pass (default; FP32 strips of 24 rows, FP64 of 16) and the two-step one
+# (--steps-per-pass 2; FP32 10 rows, FP64 12), with the profiles holding the summaries and
+# the commands
+TRAFFIC = {
+    4: ({"f32": 1_043_207_473, "f64": 2_121_719_764}, "profiles/bulkn_kernel_8192x2048_hbm_bytes.md"),
+    2: ({"f32": 949_550_578, "f64": 2_289_678_504}, "profiles/bulk2_kernel_8192x2048_hbm_bytes.md"),
+}
 NX, NY = 8192, 2048
 # HIP event pairs around every 4th bulk launch on long runs: the per-launch mean without the
 # events' own cost (about 2% of the step when recorded around every launch); runs of up to
@@ -56,7 +61,7 @@ REF_RATIO_8T, REF_RATIO_1T = (3.52, 3.73), (1.29, 2.15)
 UMAX, RE = 0.05, 200.0
 
 
-def make_lattice(precision, nx=NX, ny=NY, strip_rows=0, kernel=0, time_blocking=-1):
+def make_lattice(precision, nx=NX, ny=NY, strip_rows=0, kernel=0, time_blocking=-1, steps_per_pass=0):
     from mrlbm_2d_amd import capi
     p = capi.default_params()
     p.nx, p.ny = nx, ny
@@ -69,6 +74,7 @@ def make_lattice(precision, nx=NX, ny=NY, strip_rows=0, kernel=0, time_blocking=
     p.strip_rows = strip_rows
     p.kernel = kernel
     p.time_blocking = time_blocking
+    p.steps_per_pass = steps_per_pass
     lat = capi.Lattice(p)
     rng = np.random.default_rng(3)
     ux = rng.uniform(-1e-4, 1e-4, (nx, ny))
@@ -122,7 +128,7 @@ def run_form(precision, steps, warmup, gate_steps=0, gate_window=0, **kw):
     lat.close()
     n = info.nx * info.ny
     mlups = n * steps / (wall * 1e6)
-    # the timed launches advanced t.timed_steps steps (2 per two-step launch, 1 per single-step
+    # the timed launches advanced t.timed_steps steps (4, 2 or 1 per four-, two- or single-step
     # launch), t.timed_ms in all
     bytes_per_step = int(info.bytes_per_step)
     achieved = t.timed_steps * bytes_per_step / (t.timed_ms * 1e-3) / 1e9
@@ -134,14 +140,18 @@ def run_form(precision, steps, warmup, gate_steps=0, gate_window=0, **kw):
                 first_timed_step=it), out
 
 
-def roofline(r, kernel, traffic):
+def roofline(r, kernel, traffic, source):
     """Roofline of the dominant kernel: algorithmic bytes of the timed launches (12 moment
     words per node and step) over their summed duration, against 8 TB/s; traffic = HBM bytes
-    of one launch from the committed PMC profile (None for a launch shape not profiled)."""
+    of one launch from the committed PMC profile (None for a launch shape not profiled), and
+    traffic_frac = those bytes over the mean launch time, against 8 TB/s."""
+    tgbs = traffic / (r["launch_ms"] * 1e-3) / 1e9 if traffic else None
     return {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(r["frac"], 4), "traffic": traffic,
-            "traffic_source": (TRAFFIC_SOURCE + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; not measured in "
+            "traffic_source": (source + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; not measured in "
                                "this run)") if traffic else None,
+            "traffic_gbs": round(tgbs, 1) if tgbs else None,
+            "traffic_frac": round(tgbs / HBM_PEAK_GBS, 4) if tgbs else None,
             "kernel": kernel, "steps_per_launch": r["steps_per_launch"],
             "algorithmic_bytes_per_launch": int(round(r["steps_per_launch"] * r["bytes_per_step"])),
             "launch_ms": round(r["launch_ms"], 5), "launch_median_ms": round(r["launch_median_ms"], 5),
@@ -310,7 +320,9 @@ def main():
     ap.add_argument("--gate-window", type=int, default=50000)
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--time-blocking", type=int, default=-1,
-                    help="-1 auto (two steps per bulk pass), 0 one step per pass, 2..16 two-step strip rows")
+                    help="-1 auto (multi-step passes on), 0 one step per pass, 2..16 two-step strip rows")
+    ap.add_argument("--steps-per-pass", type=int, default=0, choices=(0, 2, 4),
+                    help="0 or 4: four steps per bulk pass where they apply (default), 2: two")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (for profiler counter scaling)")
     args = ap.parse_args()
@@ -324,7 +336,7 @@ def main():
     from mrlbm_2d_amd import capi
     gate_steps = max(args.gate_steps, 0)
     gate_window = min(args.gate_window, gate_steps)
-    kw = dict(strip_rows=args.strip_rows, time_blocking=args.time_blocking)
+    kw = dict(strip_rows=args.strip_rows, time_blocking=args.time_blocking, steps_per_pass=args.steps_per_pass)
     # FP64 first, then FP32 (the headline): each form's timed steps continue its own run
     # right after the accuracy steps
     print(f"[bench] FP64 plain form {NX}x{NY}: {gate_steps} accuracy steps (window {gate_window}), warmup "
@@ -350,14 +362,19 @@ def main():
         print("[bench] CPU baseline: oracle", file=sys.stderr, flush=True)
         cpu = cpu_baseline()
         print(f"[bench] CPU: {cpu}", file=sys.stderr, flush=True)
-    default_shape = args.strip_rows == 0 and args.time_blocking == -1
-    tb = args.time_blocking
-    if tb == -1:
-        k32, k64 = "bulk2_kernel<float,float,10,false>", "bulk2_kernel<double,double,12,false>"
-    elif tb == 0:
+    tb, spp = args.time_blocking, (4 if args.steps_per_pass in (0, 4) else 2)
+    if tb == 0:
+        spp = 1
         k32, k64 = "bulk_kernel<float,float,false,304>", "bulk_kernel<double,double,false,8>"
+    elif spp == 4:
+        k32, k64 = "bulkn_kernel<float,float,4,24>", "bulkn_kernel<double,double,4,16>"
     else:
-        k32, k64 = f"bulk2_kernel<float,float,{tb},false>", f"bulk2_kernel<double,double,{tb},false>"
+        h32, h64 = (10, 12) if tb == -1 else (tb, tb)
+        k32, k64 = f"bulk2_kernel<float,float,{h32},false>", f"bulk2_kernel<double,double,{h64},false>"
+    # measured traffic applies to the profiled launch shapes only
+    traffic, source = TRAFFIC.get(spp, ({"f32": None, "f64": None}, None))
+    if args.strip_rows != 0 or tb not in (-1,):
+        traffic = {"f32": None, "f64": None}
     out = {
         # the headline metric: MLUPS of the FP32 deviation form (value), its % of HBM peak
         # (roofline), the FP64 form beside it, the |dCd|/Cd condition (fp32_vs_fp64) and the
@@ -376,9 +393,9 @@ def main():
         "data": "synthetic (seeded +-1e-4 velocity noise)",
         "config": {"workload": "8192x2048 channel, cylinder D=128 at (1024,1025), "
                                "parabolic inlet u_max=0.05, Re=200", "nx": NX, "ny": NY},
-        "roofline": roofline(r32, k32, TRAFFIC_BYTES["f32"] if default_shape else None),
+        "roofline": roofline(r32, k32, traffic["f32"], source),
         "fp64": {"value": round(r64["mlups"], 1), "ms_per_step": round(r64["ms_per_step"], 5),
-                 "roofline": roofline(r64, k64, TRAFFIC_BYTES["f64"] if default_shape else None)},
+                 "roofline": roofline(r64, k64, traffic["f64"], source)},
         "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3),
         "fp32_vs_fp64": gate,
         # the timed steps continue each form's accuracy run from this step number

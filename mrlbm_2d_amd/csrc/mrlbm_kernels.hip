@@ -581,7 +581,9 @@ template <class S, class T, int NS> __device__ __forceinline__ bool n_step_node(
 template <class S, class T, int NS, int H> __device__ void bulk_march_n(const StepArgs<S, T> &a, int bb)
 {
     using F = typename FormOf<T>::F;
-    constexpr int W = 64 - 2 * NS, NP = H + 2 * NS, LA = 2;
+    // rows of state t loaded ahead: FP32 4, FP64 1 (FP64 at 2 spills at two waves per SIMD;
+    // both measured fastest at 8192 x 2048)
+    constexpr int W = 64 - 2 * NS, NP = H + 2 * NS, LA = sizeof(S) == 8 ? 1 : 4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = (bb % a.ncb) * 4 + wave, st = bb / a.ncb;
     if (g >= a.ncg) return;
