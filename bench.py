@@ -16,6 +16,8 @@ committed profile named in `traffic_source`, not measured in this run; below the
 bytes because the intermediate steps never leave the chip, so `frac` (algorithmic) passes 1
 and `traffic_frac` (those bytes over the same launch time, against 8 TB/s) is the share of HBM
 bandwidth the launch actually uses.
+`statistics_window`: ms per step inside the statistics window (forces, surface pressure,
+probes and u statistics recorded every step) against plain steps, both forms.
 `fp32_vs_fp64` is the accuracy condition of the FP32 figure: both forms from the same seeded
 state over 100,000 steps (mean Cd, rms Cl, spectral St over the last 50,000; velocity L2 at
 the end), run before the timed legs.  Also: the 512x128 drag adjoint (adjoint / forward
@@ -295,6 +297,47 @@ def batch_bench(nruns=256, steps=20000, window=10000, seed=0, sample=(0, 1, 2)):
             "cd_max": max(finite) if finite else None, "diverged": diverged, "single_run_check": cmp}
 
 
+def stats_window_bench(steps=400, warmup=40):
+    """Cost of a step inside the statistics window at 8192 x 2048, as the drop-in driver runs
+    it there: forces, surface pressure, four probes and the u statistics recorded every
+    step, per precision form; beside it the same lattice stepped with nothing recorded."""
+    from mrlbm_2d_amd import capi
+    out = {"workload": f"8192x2048 benchmark channel, {steps} steps after {warmup}: forces, surface pressure, "
+                       "4 probes (two on nodes the multi-step bulk writes) and u statistics every step"}
+    for name, prec in (("fp64", capi.MRLBM_FP64), ("fp32", capi.MRLBM_FP32_DEV)):
+        lat = make_lattice(prec)
+        try:
+            it = 2
+            lat.step(it, warmup)
+            it += warmup
+            lat.synchronize()
+            t0 = time.perf_counter()
+            lat.step(it, steps)
+            lat.synchronize()
+            plain = (time.perf_counter() - t0) / steps
+            it += steps
+            o = capi.default_step_opts()
+            o.post_process, o.statsbegin, o.statsend = 1, it, 10 ** 8
+            o.record_surface_p, o.stats_interval = 1, 1
+            o.nprobe = 4
+            for k, (i, j) in enumerate(((1160, 1026), (4000, 1026), (2000, 3), (8192, 700))):
+                o.probe_i[k], o.probe_j[k] = i, j
+            lat.step(it, warmup, o)
+            it += warmup
+            lat.synchronize()
+            t0 = time.perf_counter()
+            lat.step(it, steps, o)
+            lat.synchronize()
+            win = (time.perf_counter() - t0) / steps
+            assert lat.check_finite() == 0, "non-finite state in the statistics window"
+        finally:
+            lat.close()
+        out[name] = {"ms_per_step_window": round(1e3 * win, 5), "ms_per_step_plain": round(1e3 * plain, 5),
+                     "window_over_plain": round(win / plain, 3)}
+        print(f"[bench] statistics window {name}: {out[name]}", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration(nbytes, reps):
     """A plain device copy of `nbytes` (torch's copy kernel), `reps` times: the known-size
     transfer that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of a profiled run."""
@@ -315,6 +358,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-stats-window", action="store_true")
     ap.add_argument("--gate-steps", type=int, default=100000,
                     help="steps of the FP32-vs-FP64 accuracy run (0 skips it)")
     ap.add_argument("--gate-window", type=int, default=50000)
@@ -349,6 +393,10 @@ def main():
     gate = precision_gate(run64, run32, gate_steps, gate_window) if gate_steps > 0 else None
     del run64, run32
     print(f"[bench] FP32 vs FP64: {gate}", file=sys.stderr, flush=True)
+    window = None
+    if not args.no_stats_window:
+        print("[bench] statistics window", file=sys.stderr, flush=True)
+        window = stats_window_bench()
     adj = None
     if not args.no_adjoint:
         print("[bench] drag adjoint, 512x128", file=sys.stderr, flush=True)
@@ -400,6 +448,7 @@ def main():
         "fp32_vs_fp64": gate,
         # the timed steps continue each form's accuracy run from this step number
         "first_timed_step": r32["first_timed_step"],
+        "statistics_window": window,
         "adjoint": adj,
         "batched": batch,
         "cpu_baseline": cpu,

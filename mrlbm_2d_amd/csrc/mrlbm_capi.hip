@@ -744,18 +744,15 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         const int nrec = (is_window(iter) ? 1 : 0) + (is_window(iter + 1) ? 1 : 0);
         return (int)rec_iters.size() + nrec <= std::max(h->rec_cap, 0) || nrec == 0;
     };
-    // four iterations may share one four-step bulk pass when none of them samples the u
-    // statistics, records probes (the bulk carries neither) or zeroes the solid nodes' velocity,
-    // and the records fit
+    // four iterations may share one four-step bulk pass when none of them zeroes the solid
+    // nodes' velocity (plot steps) and the records fit; the bulk samples the u statistics and
+    // records the probes of the nodes it writes, the sub-steps those of the others
     auto quad_ok = [&](int iter) {
         if (!h->ms_H || !h->tb_H || h->from_pops || h->chi || h->p.kernel == 1) return false;
         int nrec = 0;
         for (int it = iter; it <= iter + 3; it++) {
-            if (plot_step(it) || sample_step(it)) return false;
-            if (is_window(it)) {
-                if (opts.nprobe > 0) return false;
-                nrec++;
-            }
+            if (plot_step(it)) return false;
+            if (is_window(it)) nrec++;
         }
         return (int)rec_iters.size() + nrec <= std::max(h->rec_cap, 0) || nrec == 0;
     };
@@ -824,13 +821,26 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             bulk.ncg = h->ncgN;
             bulk.ncb = h->ncbN;
             bulk.nstrips = h->nstripsN;
+            const int slot0 = (int)rec_iters.size();
+            for (int k = 0, slot = slot0; k < 4; k++) { // the bulk's samples and probe records
+                bulk.qsample |= sample_step(iter + k) ? 1 << k : 0;
+                bulk.qslot[k] = is_window(iter + k) ? slot++ : -1;
+            }
             HIPCHK(hipStreamWaitEvent(h->st2, h->bulk_done, 0));
             HIPCHK(hipStreamWaitEvent(h->st, h->evB, 0));
             HIPCHK(launch_bulk_quad(bulk));
-            int slot = (int)rec_iters.size();
+            if (opts.nprobe > 0 && std::any_of(bulk.qslot, bulk.qslot + 4, [](int q) { return q >= 0; })) {
+                // the probes on the bulk's nodes; the next pass's side work (which overwrites
+                // state t's band nodes) then waits for this kernel too
+                HIPCHK(launch_quad_probes<S, T>(bulk, h->st));
+                HIPCHK(hipEventRecord(h->evA, h->st));
+                h->bulk_done = h->evA;
+            }
+            int slot = slot0;
             for (int k = 0; k < 4; k++) {
                 auto a = step_args(iter + k, src[k], dst[k], slot);
                 if (is_window(iter + k)) slot++;
+                a.skip_ns = 4;
                 a.nrect = h->nrectq[k];
                 std::memcpy(a.rect, h->rectq[k], sizeof h->rectq[k]);
                 HIPCHK(side_work(a));

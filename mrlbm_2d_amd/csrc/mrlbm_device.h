@@ -204,6 +204,22 @@ __device__ void edge_update(const StepArgs<S, T> &a, int c, int j, const T fpop[
     m[0] = rho_written ? F::store_rho(mm[0]) : d_keep;
 }
 
+// Output nodes of an NS-step pass: NS or more nodes from the outer ring and outside the
+// cylinder box dilated by NS-1 (everything else is finished by the band sub-steps).
+template <class S, class T, int NS> __device__ __forceinline__ bool n_step_node(const StepArgs<S, T> &a, int c, int j)
+{
+    if (c < NS || c > a.nx - 1 - NS || j < NS || j > a.ny - 1 - NS) return false;
+    return !(a.bw > 0 && c >= a.bx0 - (NS - 1) && c <= a.bx0 + a.bw + (NS - 2) && j >= a.by0 - (NS - 1) &&
+             j <= a.by0 + a.bh + (NS - 2));
+}
+
+// Whether this sub-step records the samples and probes of node (c, j): in a four-step pass
+// (skip_ns = 4) the four-step bulk records those of the nodes it writes, at every level.
+template <class S, class T> __device__ __forceinline__ bool records_node(const StepArgs<S, T> &a, int c, int j)
+{
+    return a.skip_ns != 4 || !n_step_node<S, T, 4>(a, c, j);
+}
+
 // Everything after the gather for one bulk node: moments, edge BC, the Brinkman
 // penalization u <- (1 - chi) u of non-edge nodes (CHI, not in the reference), solid-node
 // zeroing on plot steps (2d_mrlbm.f90:375-384), recording, collision, store.
@@ -232,7 +248,8 @@ __device__ __forceinline__ void node_update(const StepArgs<S, T> &a, int c, int 
         m[2] = (T)0.0;
     }
     if (a.inlet_out && c == 0) a.inlet_out[j] = F::dev ? 1.0 + (double)m[0] : (double)m[0];
-    if (a.rec_slot >= 0) {
+    const bool own = records_node(a, c, j);
+    if (a.rec_slot >= 0 && own) {
         for (int p = 0; p < a.nprobe; p++)
             if (n == a.probe_node[p]) {
                 double *r = a.rec + (long long)a.rec_slot * a.rec_stride + 2 + 3 * p;
@@ -241,7 +258,7 @@ __device__ __forceinline__ void node_update(const StepArgs<S, T> &a, int c, int 
                 r[2] = m[2];
             }
     }
-    if (a.sample) {
+    if (a.sample && own) {
         const double ux = m[1], uy = m[2];
         a.acc[n] += ux;
         a.acc[a.plane + n] += uy;

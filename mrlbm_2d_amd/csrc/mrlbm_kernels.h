@@ -47,7 +47,11 @@ template <class S, class T> struct StepArgs {
     // recording
     int sample;                // accumulate mean/rms u (two-step bulk: of t+1)
     int sample2;               // two-step bulk: accumulate those of t+2
-    double *acc;               // 5 planes: ux, uy, ux^2, uy^2, ux*uy
+    int qsample;               // four-step bulk: bit s-1 set = accumulate those of t+s
+    int qslot[4];              // four-step bulk: record slot of t+s for the probes, or -1
+    int skip_ns;               // sub-step of a four-step pass: leave the samples and probe records of
+                               // the nodes the four-step bulk writes (n_step_node, NS = skip_ns) to it
+    double *acc;               // 4 planes: ux, uy, ux^2, uy^2
     int rec_slot;              // >= 0: record forces/probes/surface p into rec + slot*rec_stride
     double *rec;
     int rec_stride, record_p;
@@ -90,6 +94,10 @@ hipError_t launch_pair_probes(const StepArgs<S, T> &a, const S *s2, int slot2, h
 // rectangles that the two-step bulk writes (a: sub-step 1; s2: state t+2); after both.
 template <class S, class T>
 hipError_t launch_pair_samples(const StepArgs<S, T> &a, const S *s2, hipStream_t st);
+// Probes of a four-step pass on the nodes the four-step bulk writes (a: the bulk's arguments,
+// with qslot), after the bulk: t+1 .. t+3 recomputed from state t, t+4 from the stored state.
+template <class S, class T>
+hipError_t launch_quad_probes(const StepArgs<S, T> &a, hipStream_t st);
 // Largest ux (NaN ignored) and a non-finite flag (1.0 or 0.0) over the six stored moments
 // of the nx x ny nodes of one state, into scratch[0] and scratch[1]; the rest of the
 // scratch_words doubles holds per-block partial results.

@@ -558,15 +558,6 @@ __global__ void __launch_bounds__(256) bulk2_kernel(StepArgs<S, T> a)
     bulk_march2<S, T, H, SAMPLE>(a, blockIdx.x);
 }
 
-// Output nodes of an NS-step pass: NS or more nodes from the outer ring and outside the
-// cylinder box dilated by NS-1 (everything else is finished by the band sub-steps).
-template <class S, class T, int NS> __device__ __forceinline__ bool n_step_node(const StepArgs<S, T> &a, int c, int j)
-{
-    if (c < NS || c > a.nx - 1 - NS || j < NS || j > a.ny - 1 - NS) return false;
-    return !(a.bw > 0 && c >= a.bx0 - (NS - 1) && c <= a.bx0 + a.bw + (NS - 2) && j >= a.by0 - (NS - 1) &&
-             j <= a.by0 + a.bh + (NS - 2));
-}
-
 // NS steps in one pass over the plain interior, src = state t, dst = state t+NS.  A wave holds
 // 64 consecutive columns and writes the middle 64 - 2 NS of them: state t+s computed in all
 // lanes is right in lanes s .. 63-s (each level loses one column per side to the lanes whose
@@ -578,7 +569,12 @@ template <class S, class T, int NS> __device__ __forceinline__ bool n_step_node(
 // above, level NS stores its moments.  Level s makes the rows j0-NS+s .. j0+H+NS-1-s, the
 // first of them at march step 2s.  Per node and level the arithmetic is the single step's, so
 // the result is bit-identical to NS single steps.
-template <class S, class T, int NS, int H> __device__ void bulk_march_n(const StepArgs<S, T> &a, int bb)
+// REC: also accumulate the u statistics (levels with a bit in a.qsample) and record the probes
+// (levels with a.qslot >= 0) of the nodes the pass writes, at every level, as single steps
+// would; the sub-steps of the pass leave those nodes to it (records_node).  A node's level-s
+// velocity is made NS-s march steps before its level-NS one, so the wave carries it that
+// long, then adds the sampled levels in order in one read-modify-write of the accumulators.
+template <class S, class T, int NS, int H, bool REC> __device__ void bulk_march_n(const StepArgs<S, T> &a, int bb)
 {
     using F = typename FormOf<T>::F;
     // rows of state t loaded ahead: FP32 4, FP64 1 (FP64 at 2 spills at two waves per SIMD;
@@ -602,6 +598,16 @@ template <class S, class T, int NS, int H> __device__ void bulk_march_n(const St
 #pragma unroll
     for (int i = 0; i <= LA; i++) load_row(i, q[i]);
     T upp[NS + 1][3], upc[NS + 1][3], mid[NS + 1][3];
+    // REC: velocities of levels 1 .. NS-1, a ring of NS-s+1 entries per level s.  FP32 keeps
+    // them in registers (indices are compile-time once the march is unrolled; NS + 1 rows so
+    // that hu[l] stays in bounds where a loop over l = 1 .. NS names it); FP64, already at the
+    // register limit of two waves per SIMD, in LDS: per wave, level s at ring entries
+    // base(s) = sum over l < s of (NS-l+1), 2 values of 64 lanes each.
+    constexpr bool DL_LDS = REC && sizeof(S) == 8;
+    constexpr int DLN = NS * (NS + 1) / 2 - 1; // ring entries of levels 1 .. NS-1
+    T hu[NS + 1][NS + 1], hv[NS + 1][NS + 1];
+    __shared__ T dl[DL_LDS ? 4 : 1][DL_LDS ? DLN : 1][2][64];
+    auto dl_at = [](int l, int e) { return (l - 1) * (NS + 1) - (l - 1) * l / 2 + e; }; // base(l) + e
 #pragma unroll
     for (int p = 0; p < NP; p++) {
         T Fin[Q]; // populations of the newest row of the level below
@@ -633,6 +639,17 @@ template <class S, class T, int NS, int H> __device__ void bulk_march_n(const St
             if (!make) break; // its window fills at steps 2s-2 and 2s-1
             T m[6];
             F::moments(f, m);
+            if constexpr (REC) {
+                if (s < NS) {
+                    if constexpr (DL_LDS) {
+                        dl[wave][dl_at(s, p % (NS - s + 1))][0][lane] = m[1];
+                        dl[wave][dl_at(s, p % (NS - s + 1))][1][lane] = m[2];
+                    } else {
+                        hu[s][p % (NS - s + 1)] = m[1];
+                        hv[s][p % (NS - s + 1)] = m[2];
+                    }
+                }
+            }
             collide<T>(m, a.om, a.om1, a.incomp_bulk);
             if (s < NS) {
                 F::recon(m, Fin);
@@ -642,6 +659,36 @@ template <class S, class T, int NS, int H> __device__ void bulk_march_n(const St
                     const long long n = (long long)j * a.pitch + c;
 #pragma unroll
                     for (int k = 0; k < 6; k++) dst[k * plane + n] = (S)m[k];
+                    if constexpr (REC) {
+                        if (a.qsample) {
+                            double s0 = a.acc[n], s1 = a.acc[plane + n], s2 = a.acc[2 * plane + n],
+                                   s3 = a.acc[3 * plane + n];
+#pragma unroll
+                            for (int l = 1; l <= NS; l++) {
+                                if (!((a.qsample >> (l - 1)) & 1)) continue;
+                                // level l made this row NS - l march steps ago
+                                const int e = (p - (NS - l)) % (NS - l + 1);
+                                double x = m[1], y = m[2];
+                                if (l < NS) {
+                                    if constexpr (DL_LDS) {
+                                        x = dl[wave][dl_at(l, e)][0][lane];
+                                        y = dl[wave][dl_at(l, e)][1][lane];
+                                    } else {
+                                        x = hu[l][e];
+                                        y = hv[l][e];
+                                    }
+                                }
+                                s0 += x;
+                                s1 += y;
+                                s2 += x * x;
+                                s3 += y * y;
+                            }
+                            a.acc[n] = s0;
+                            a.acc[plane + n] = s1;
+                            a.acc[2 * plane + n] = s2;
+                            a.acc[3 * plane + n] = s3;
+                        }
+                    }
                 }
             }
         }
@@ -649,10 +696,10 @@ template <class S, class T, int NS, int H> __device__ void bulk_march_n(const St
 }
 
 // at least two waves per SIMD (left alone, the FP64 kernel takes 256 VGPRs plus AGPRs: one)
-template <class S, class T, int NS, int H>
+template <class S, class T, int NS, int H, bool REC = false>
 __global__ void __launch_bounds__(256, 2) bulkn_kernel(StepArgs<S, T> a)
 {
-    bulk_march_n<S, T, NS, H>(a, blockIdx.x);
+    bulk_march_n<S, T, NS, H, REC>(a, blockIdx.x);
 }
 
 // The side stream's node work of one step in one launch: the outer ring (edge update),
@@ -1221,6 +1268,57 @@ template <class S, class T> __global__ void __launch_bounds__(256) pair_samples_
     }
 }
 
+// Probes of a four-step pass on the nodes the four-step bulk writes, one workgroup per
+// probe, after the bulk.  The bulk carries t+1 .. t+3 in registers, so this recomputes them
+// at the probe node from state t: level l = 1..3 on the (7-2l)^2 nodes around it, each a
+// plain interior update with the single step's arithmetic (such a node's cone of radius 3
+// lies in the plain interior: it is four or more from the ring and outside the cylinder box
+// dilated by three), then t+4 from the stored state.  a: the bulk's arguments (src = state
+// t, dst = state t+4, qslot = the record slot of t+1 .. t+4 or -1).
+template <class S, class T> __global__ void __launch_bounds__(64) quad_probes_kernel(StepArgs<S, T> a)
+{
+    using F = typename FormOf<T>::F;
+    const int p = blockIdx.x, t = threadIdx.x;
+    const long long n0 = a.probe_node[p];
+    const int c = (int)(n0 % a.pitch), j = (int)(n0 / a.pitch);
+    if (!n_step_node<S, T, 4>(a, c, j)) return; // the sub-steps record it (uniform per block)
+    __shared__ T mom[49][6];  // moments of the level below, on its (9-2l)^2 nodes
+    __shared__ T pop[49][Q];  // their populations
+    auto record = [&](int slot, T r0, T u, T v) {
+        double *r = a.rec + (long long)slot * a.rec_stride + 2 + 3 * p;
+        r[0] = F::dev ? 1.0 + (double)r0 : (double)r0;
+        r[1] = u;
+        r[2] = v;
+    };
+    if (t < 49) { // level 0: state t at the 7 x 7 nodes around the probe
+        const long long n = (long long)(j - 3 + t / 7) * a.pitch + (c - 3 + t % 7);
+#pragma unroll
+        for (int k = 0; k < 6; k++) mom[t][k] = a.src[k * a.plane + n];
+    }
+    __syncthreads();
+    for (int l = 1; l <= 3; l++) {
+        const int wb = 9 - 2 * l, w = wb - 2; // widths of the level below and of level l
+        if (t < wb * wb) F::recon(mom[t], pop[t]);
+        __syncthreads();
+        T m[6];
+        if (t < w * w) {
+            const int x = t % w + 1, y = t / w + 1; // in the level below's coordinates
+            T f[Q];
+#pragma unroll
+            for (int k = 0; k < Q; k++) f[k] = pop[(y - EYk(k)) * wb + (x - EXk(k))][k];
+            F::moments(f, m);
+            if (t == w * w / 2 && a.qslot[l - 1] >= 0) record(a.qslot[l - 1], m[0], m[1], m[2]);
+            collide<T>(m, a.om, a.om1, a.incomp_bulk);
+        }
+        __syncthreads(); // every read of mom by recon is done
+        if (t < w * w)
+#pragma unroll
+            for (int k = 0; k < 6; k++) mom[t][k] = m[k];
+        __syncthreads();
+    }
+    if (t == 0 && a.qslot[3] >= 0) record(a.qslot[3], (T)a.dst[n0], (T)a.dst[a.plane + n0], (T)a.dst[2 * a.plane + n0]);
+}
+
 // A pitched state plane to a dense (nx, ny) FP64 array (i fastest) and back, shifted by
 // `shift` (the FP32 form stores rho - 1): the host copies of the C-ABI are then one
 // contiguous transfer.  Padding columns of the pitched plane are written as zeros.
@@ -1492,15 +1590,25 @@ template <class S, class T>
 hipError_t launch_bulkn(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0, hipEvent_t t1)
 {
     const dim3 g(nblocks), b(256);
-    // strip heights: FP32 24 (12 on small grids), FP64 16 (8)
+    // strip heights: FP32 24 (12 on small grids), FP64 16 (8); the recording variant when a
+    // level is sampled
+    const bool rec = a.qsample != 0;
     if constexpr (sizeof(S) == 4) {
-        if (H == 24) launch_timed(bulkn_kernel<S, T, 4, 24>, g, b, st, t0, t1, a);
-        else if (H == 12) launch_timed(bulkn_kernel<S, T, 4, 12>, g, b, st, t0, t1, a);
-        else return hipErrorInvalidValue;
+        if (H == 24) {
+            if (rec) launch_timed(bulkn_kernel<S, T, 4, 24, true>, g, b, st, t0, t1, a);
+            else launch_timed(bulkn_kernel<S, T, 4, 24>, g, b, st, t0, t1, a);
+        } else if (H == 12) {
+            if (rec) launch_timed(bulkn_kernel<S, T, 4, 12, true>, g, b, st, t0, t1, a);
+            else launch_timed(bulkn_kernel<S, T, 4, 12>, g, b, st, t0, t1, a);
+        } else return hipErrorInvalidValue;
     } else {
-        if (H == 16) launch_timed(bulkn_kernel<S, T, 4, 16>, g, b, st, t0, t1, a);
-        else if (H == 8) launch_timed(bulkn_kernel<S, T, 4, 8>, g, b, st, t0, t1, a);
-        else return hipErrorInvalidValue;
+        if (H == 16) {
+            if (rec) launch_timed(bulkn_kernel<S, T, 4, 16, true>, g, b, st, t0, t1, a);
+            else launch_timed(bulkn_kernel<S, T, 4, 16>, g, b, st, t0, t1, a);
+        } else if (H == 8) {
+            if (rec) launch_timed(bulkn_kernel<S, T, 4, 8, true>, g, b, st, t0, t1, a);
+            else launch_timed(bulkn_kernel<S, T, 4, 8>, g, b, st, t0, t1, a);
+        } else return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -1555,6 +1663,11 @@ template <class S, class T>
 hipError_t launch_pair_probes(const StepArgs<S, T> &a, const S *s2, int slot2, hipStream_t st)
 {
     hipLaunchKernelGGL((pair_probes_kernel<S, T>), dim3(1), dim3(64), 0, st, a, s2, slot2);
+    return hipGetLastError();
+}
+template <class S, class T> hipError_t launch_quad_probes(const StepArgs<S, T> &a, hipStream_t st)
+{
+    if (a.nprobe > 0) hipLaunchKernelGGL((quad_probes_kernel<S, T>), dim3(a.nprobe), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 template <class S>
@@ -1637,6 +1750,7 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_inlet_fold<S, T>(const StepArgs<S, T> &, hipStream_t);            \
     template hipError_t launch_pair_probes<S, T>(const StepArgs<S, T> &, const S *, int, hipStream_t); \
     template hipError_t launch_pair_samples<S, T>(const StepArgs<S, T> &, const S *, hipStream_t); \
+    template hipError_t launch_quad_probes<S, T>(const StepArgs<S, T> &, hipStream_t);            \
     template hipError_t launch_get_populations<S, T>(const StepArgs<S, T> &, double *, int, int, hipStream_t); \
     template hipError_t launch_ghost_from_state<S, T>(const StepArgs<S, T> &, T *, hipStream_t);  \
     template hipError_t launch_ghost_from_pops<S, T>(const StepArgs<S, T> &, T *, hipStream_t);

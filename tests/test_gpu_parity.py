@@ -235,22 +235,26 @@ def test_400x100_channel_20000_steps_matches_oracle(oracle_mod):
     # row 2 (band at t+1, bulk at t+2), column 1 (band), the outlet column (ring), bulk
     ("p1_walls", ((60, 3), (2, 20), (88, 20), (60, 20))),
     ("s_xyper_noise", ((40, 3), (2, 20), (56, 20), (45, 20))),
+    # nodes the four-step bulk writes that its first sub-steps also write (column 4, the
+    # band rows near the top, column nx-5) and one that only the bulk writes
+    ("p1_walls", ((5, 20), (60, 34), (84, 20), (8, 30))),
 ])
-def test_probes_in_two_step_pairs_match_single_steps(name, probes, precision):
-    """Probe records inside two-step pairs (nodes the bulk carries through t+1 recorded by
-    their own kernel, the rest by the side kernels) equal those of single steps, bit for bit,
-    with the statistics window opening and closing inside a pair."""
+def test_probes_in_multi_step_passes_match_single_steps(name, probes, precision):
+    """Probe records inside two- and four-step passes (nodes the bulk carries between its
+    stores recorded by their own kernel, the rest by the side kernels) equal those of single
+    steps, bit for bit, with the statistics window opening and closing inside a pass."""
     g = gc.load(name)
     p = dict(g["params"])
     p.update(post_process=True, statsbegin=7, statsend=50)
     out = {}
-    for tb in (0, -1):
+    for spp in (1, 4, 2):  # steps per pass: single steps, four-step passes, two-step pairs
         par = params_from_namelist(p, precision=precision)
-        par.time_blocking = tb
+        par.time_blocking = 0 if spp == 1 else -1
+        par.steps_per_pass = 2 if spp == 2 else 0
         with Lattice(par) as lat:
             if "f0" in g:
                 lat.set_populations(g["f0"])
-            print(f"[gpu] probes {name} {probes} precision={precision} time_blocking={tb}", flush=True)
+            print(f"[gpu] probes {name} {probes} precision={precision} steps per pass {spp}", flush=True)
             o = step_opts_from_namelist(p)
             o.nprobe = len(probes)
             for k, (i, j) in enumerate(probes):
@@ -259,42 +263,45 @@ def test_probes_in_two_step_pairs_match_single_steps(name, probes, precision):
             lat.step(2, 60, o)
             _, launches = lat.last_bulk_ms()
             it, v = lat.probes()
-            out[tb] = (launches, it, v, lat.fields())
-    assert out[0][0] == 60 and out[-1][0] < 40, (out[0][0], out[-1][0])  # bulk launches
-    np.testing.assert_array_equal(out[-1][1], np.arange(7, 51, dtype=np.int32))
-    np.testing.assert_array_equal(out[-1][1], out[0][1])
-    np.testing.assert_array_equal(out[-1][2], out[0][2])
-    for a, b in zip(out[-1][3], out[0][3]):
-        np.testing.assert_array_equal(a, b)
+            out[spp] = (launches, it, v, lat.fields())
+    assert out[1][0] == 60 and out[4][0] < 20 and out[2][0] < 40, [out[k][0] for k in out]  # bulk launches
+    np.testing.assert_array_equal(out[1][1], np.arange(7, 51, dtype=np.int32))
+    for spp in (4, 2):
+        np.testing.assert_array_equal(out[spp][1], out[1][1])
+        np.testing.assert_array_equal(out[spp][2], out[1][2])
+        for a, b in zip(out[spp][3], out[1][3]):
+            np.testing.assert_array_equal(a, b)
 
 
 @pytest.mark.parametrize("precision", [capi.MRLBM_FP64, capi.MRLBM_FP32_DEV])
-@pytest.mark.parametrize("interval", [1, 3])
-def test_u_statistics_in_two_step_pairs_match_single_steps(oracle_mod, precision, interval):
-    """Mean / rms velocity accumulators sampled inside two-step pairs (the bulk accumulates
-    the nodes it carries through t+1 and writes at t+2, the side kernels the rest) equal
-    those of single steps bit for bit; in FP64 the means also equal the oracle's fields
-    averaged over the same steps."""
+@pytest.mark.parametrize("interval", [1, 2, 3])
+def test_u_statistics_in_multi_step_passes_match_single_steps(oracle_mod, precision, interval):
+    """Mean / rms velocity accumulators sampled inside four-step passes and two-step pairs
+    (the bulk accumulates the sampled levels of the nodes it writes, the side kernels the
+    rest) equal those of single steps bit for bit; in FP64 the means also equal the oracle's
+    fields averaged over the same steps."""
     g = gc.load("p1_walls")
     p = dict(g["params"])
     p.update(post_process=True, statsbegin=5, statsend=60)
     out = {}
-    for tb in (0, -1):
+    for spp in (1, 4, 2):  # steps per pass: single steps, four-step passes, two-step pairs
         par = params_from_namelist(p, precision=precision)
-        par.time_blocking = tb
+        par.time_blocking = 0 if spp == 1 else -1
+        par.steps_per_pass = 2 if spp == 2 else 0
         with Lattice(par) as lat:
-            print(f"[gpu] u statistics precision={precision} interval={interval} time_blocking={tb}", flush=True)
+            print(f"[gpu] u statistics precision={precision} interval={interval} steps per pass {spp}", flush=True)
             o = step_opts_from_namelist(p)
             o.stats_interval = interval
             lat.set_timing(1)
             lat.step(2, 70, o)
             _, launches = lat.last_bulk_ms()
-            out[tb] = (launches, lat.stats(), lat.fields())
-    assert out[0][0] == 70 and out[-1][0] < 50, (out[0][0], out[-1][0])  # bulk launches
-    for a, b in zip(out[-1][1], out[0][1]):
-        np.testing.assert_array_equal(a, b)
-    for a, b in zip(out[-1][2], out[0][2]):
-        np.testing.assert_array_equal(a, b)
+            out[spp] = (launches, lat.stats(), lat.fields())
+    assert out[1][0] == 70 and out[4][0] < 20 and out[2][0] < 50, [out[k][0] for k in out]  # bulk launches
+    for spp in (4, 2):
+        for a, b in zip(out[spp][1], out[1][1]):
+            np.testing.assert_array_equal(a, b)
+        for a, b in zip(out[spp][2], out[1][2]):
+            np.testing.assert_array_equal(a, b)
     if precision == capi.MRLBM_FP64:
         orc = oracle_mod.Oracle(gc.oracle_params(oracle_mod, p))
         su, sv, n = 0.0, 0.0, 0
@@ -303,17 +310,17 @@ def test_u_statistics_in_two_step_pairs_match_single_steps(oracle_mod, precision
             if 5 <= it <= 60 and (it - 5) % interval == 0:
                 _, ux, uy, *_ = orc.moments()
                 su, sv, n = su + ux, sv + uy, n + 1
-        mean_ux, mean_uy, _, _, ns = out[-1][1]
+        mean_ux, mean_uy, _, _, ns = out[4][1]
         assert ns == n
         np.testing.assert_allclose(mean_ux, su / n, rtol=1e-13, atol=1e-17)
         np.testing.assert_allclose(mean_uy, sv / n, rtol=1e-13, atol=1e-17)
 
 
-def test_full_size_recording_in_pairs_matches_single_steps():
+def test_full_size_recording_in_passes_matches_single_steps():
     """8192 x 2048 FP64 (the benchmark channel, D = 128) with everything recorded: forces,
     surface pressure, four probes and u statistics every 3rd step inside a window that
-    opens and closes mid-pair.  Two steps per pass against one step per pass: fields,
-    force, probe and surface-pressure series and the statistics bit for bit."""
+    opens and closes mid-pass.  Four (and two) steps per pass against one step per pass:
+    fields, force, probe and surface-pressure series and the statistics bit for bit."""
     import bench
     out = {}
     for tb in (0, -1):
@@ -324,7 +331,7 @@ def test_full_size_recording_in_pairs_matches_single_steps():
             o.post_process, o.statsbegin, o.statsend = 1, 41, 220
             o.record_surface_p, o.stats_interval = 1, 3
             o.nprobe = 4
-            for k, (i, j) in enumerate(((1160, 1026), (1300, 1026), (2000, 3), (8192, 700))):
+            for k, (i, j) in enumerate(((1160, 1026), (1094, 1026), (2000, 3), (8192, 700))):
                 o.probe_i[k], o.probe_j[k] = i, j
             lat.step(2, 240, o)
             out[tb] = (lat.fields(), lat.forces(), lat.probes(), lat.surface_p_series(), lat.stats())
