@@ -251,10 +251,10 @@ def batch_bench(nruns=256, steps=20000, window=10000, seed=0, sample=(0, 1, 2)):
     setup_s = time.perf_counter() - t0
     info = b.info
     tune = [{"kernel": info.tune_kernel[i], "persistent": info.tune_persistent[i], "time_blocking": info.tune_tb[i],
-             "ms_per_step": round(info.tune_ms[i], 4)} for i in range(info.ntune)]
+             "quad_rows": info.tune_quad[i], "ms_per_step": round(info.tune_ms[i], 4)} for i in range(info.ntune)]
     print(f"[bench] batch: {nruns} runs, {info.nodes} nodes, kernel {info.kernel} persistent "
-          f"{info.persistent} time_blocking {info.time_blocking}, setup+autotune {setup_s:.1f} s",
-          file=sys.stderr, flush=True)
+          f"{info.persistent} time_blocking {info.time_blocking} four-step rows {info.quad_rows}, "
+          f"setup+autotune {setup_s:.1f} s", file=sys.stderr, flush=True)
     for r in range(nruns):
         BC.seed_state(b.run(r), seed, r)
     t0 = time.perf_counter()
@@ -291,7 +291,7 @@ def batch_bench(nruns=256, steps=20000, window=10000, seed=0, sample=(0, 1, 2)):
                         f"Cd over the last {window}",
             "nodes": int(info.nodes), "value": round(info.nodes * steps / wall / 1e6, 1), "unit": "MLUPS",
             "wall_s": round(wall, 3), "kernel": info.kernel, "persistent": info.persistent,
-            "time_blocking": info.time_blocking,
+            "time_blocking": info.time_blocking, "quad_rows": info.quad_rows,
             "graph_steps": info.graph_steps, "autotune": tune, "setup_s": round(setup_s, 2),
             "cd_finite_runs": len(finite), "cd_min": min(finite) if finite else None,
             "cd_max": max(finite) if finite else None, "diverged": diverged, "single_run_check": cmp}

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define MRLBM_ABI_VERSION 1
+#define MRLBM_ABI_VERSION 2
 
 enum {
     MRLBM_OK = 0,
@@ -232,7 +232,10 @@ typedef struct mrlbm_batch_opts {
     int32_t tune_steps;           /* timed steps per autotune candidate (0 = 20) */
     int32_t time_blocking;        /* -1 autotune, 0 one step per pass, 2..10 (even) two steps per pass
                                      with strips of that many rows (as mrlbm_params) */
-    int32_t reserved[3];
+    int32_t steps_per_pass;       /* 0: autotune over one, two and four steps per bulk pass;
+                                     2: at most two; 4: four (strip height autotuned; pairs or single
+                                     steps for the rest of a graph) */
+    int32_t reserved[2];
 } mrlbm_batch_opts;
 
 typedef struct mrlbm_batch_info {
@@ -242,6 +245,8 @@ typedef struct mrlbm_batch_info {
     int32_t ntune;                /* autotune candidates timed */
     int32_t tune_kernel[16], tune_persistent[16], tune_tb[16];
     double tune_ms[16];           /* mean step time of each candidate */
+    int32_t quad_rows;            /* chosen four-step strip height, 0 = no four-step passes */
+    int32_t tune_quad[16];        /* four-step strip height of each candidate, 0 = none */
 } mrlbm_batch_info;
 
 void mrlbm_default_batch_opts(mrlbm_batch_opts *o);

@@ -96,7 +96,8 @@ class AdjointResult(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [
         ("kernel", C.c_int32), ("persistent", C.c_int32), ("graph_steps", C.c_int32),
-        ("tune_steps", C.c_int32), ("time_blocking", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("tune_steps", C.c_int32), ("time_blocking", C.c_int32), ("steps_per_pass", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -106,7 +107,7 @@ class BatchInfo(C.Structure):
         ("ntiles", C.c_int32), ("graph_steps", C.c_int32), ("nodes", C.c_int64),
         ("time_blocking", C.c_int32), ("ntune", C.c_int32),
         ("tune_kernel", C.c_int32 * 16), ("tune_persistent", C.c_int32 * 16), ("tune_tb", C.c_int32 * 16),
-        ("tune_ms", C.c_double * 16),
+        ("tune_ms", C.c_double * 16), ("quad_rows", C.c_int32), ("tune_quad", C.c_int32 * 16),
     ]
 
 

@@ -8,8 +8,10 @@
 // joined as in the single-run step loop.  Runs of graph_steps steps are captured once per
 // (start parity, recording mode) and replayed.  With two steps per pass, a pair of steps is
 // the two-step bulk over its own tile table plus two band sub-steps on the second stream,
-// as in the single-run loop.  The autotuner times the candidate bulk variants, grid modes
-// and two-step strip heights on a saved copy of the states and keeps the fastest.
+// as in the single-run loop; with four steps per pass, the four-step bulk over its tile table
+// plus four band sub-steps through the runs' two scratch states.  The autotuner times the
+// candidate bulk variants, grid modes and two- and four-step strip heights on a saved copy
+// of the states and keeps the fastest.
 #include "mrlbm_handle.h"
 #include "mrlbm_lattice.h"
 
@@ -30,6 +32,14 @@ struct mrlbm_batch {
     int kernel = 0, persistent = 1, grid = 0, ntiles = 0, graph_steps = 50, tune_steps = 20;
     int tb = 0, ntiles2 = 0;      // two steps per bulk pass: strip height (0 off), tiles
     bool tb_possible = true;      // every run has the two-step scratch state
+    int qh = 0, ntiles4 = 0;      // four steps per bulk pass: strip height (0 off), tiles
+    bool q_possible = true;       // every run has the four-step scratch states and rectangles
+    int spp = 0;                  // steps_per_pass option: 0 autotune, 2 at most two, 4 four
+    int2 *d_tiles4 = nullptr;
+    int2 *d_qtiles[4] = {nullptr, nullptr, nullptr, nullptr}; // side blocks of sub-steps 1..4
+    int nqtiles[4] = {0, 0, 0, 0};
+    void *d_b4[2] = {nullptr, nullptr};                        // four-step bulk [parity]
+    void *d_q[4][2][2] = {};                                   // sub-steps 1..4 [parity][record]
     int2 *d_stiles[3] = {nullptr, nullptr, nullptr}; // side blocks (run, block): single step, sub-step 1, 2
     int nstiles[3] = {0, 0, 0};
     int2 *d_wtiles = nullptr;     // wall blocks (run, block of 64 boundary nodes)
@@ -46,7 +56,7 @@ struct mrlbm_batch {
     double *d_facc = nullptr; // 4 per run
     hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     int exec_flip = 0;        // parity change of one graph replay
-    std::vector<int> tune_kernel, tune_persistent, tune_tb;
+    std::vector<int> tune_kernel, tune_persistent, tune_tb, tune_quad;
     std::vector<double> tune_ms;
     std::string err;
 };
@@ -93,9 +103,9 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
 {
     drop_graphs(b);
     const int R = (int)b->runs.size();
-    std::vector<StepArgs<S, T>> args[2][2], b2[2], s1[2][2], s2[2][2];
-    std::vector<int2> tiles, tiles2;
-    std::vector<int2> stiles[3], wtiles;
+    std::vector<StepArgs<S, T>> args[2][2], b2[2], s1[2][2], s2[2][2], b4[2], q[4][2][2];
+    std::vector<int2> tiles, tiles2, tiles4;
+    std::vector<int2> stiles[3], wtiles, qtiles[4];
     b->wall_levels = false;
     auto add_blocks = [](std::vector<int2> &v, int r, int n) {
         for (int k = 0; k < (n + 63) / 64; k++) v.push_back(make_int2(r, k));
@@ -147,6 +157,25 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
                         b2[par].push_back(z);
                     }
                 }
+                if (b->qh) { // sub-steps 1..4: state t -> S2 -> S3 -> S2 -> state t+4
+                    const void *src[4] = {h->S[par], h->S2, h->S3, h->S2};
+                    void *dst[4] = {h->S2, h->S3, h->S2, h->S[par ^ 1]};
+                    for (int k = 0; k < 4; k++) {
+                        auto x = a;
+                        x.src = (const S *)src[k];
+                        x.dst = (S *)dst[k];
+                        x.nrect = h->nrectq[k];
+                        std::memcpy(x.rect, h->rectq[k], sizeof h->rectq[k]);
+                        q[k][par][rec].push_back(x);
+                    }
+                    if (rec == 0) {
+                        auto z = one; // four-step bulk: state t -> state t+4
+                        z.nstrips = (h->ny - 8 + b->qh - 1) / b->qh;
+                        z.ncg = h->ncgN;
+                        z.ncb = h->ncbN;
+                        b4[par].push_back(z);
+                    }
+                }
             }
         const int nb = h->ncb * h->nstrips;
         for (int t = 0; t < nb; t++) tiles.push_back(make_int2(r, t));
@@ -156,6 +185,11 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
             const int nb2 = h->ncb2 * ((h->ny - 4 + b->tb - 1) / b->tb);
             for (int t = 0; t < nb2; t++) tiles2.push_back(make_int2(r, t));
         }
+        if (b->qh) {
+            for (int k = 0; k < 4; k++) add_blocks(qtiles[k], r, ring + band(h->rectq[k], h->nrectq[k]) + nU);
+            const int nb4 = h->ncbN * ((h->ny - 8 + b->qh - 1) / b->qh);
+            for (int t = 0; t < nb4; t++) tiles4.push_back(make_int2(r, t));
+        }
     }
     int rc = 0;
     for (int par = 0; par < 2 && !rc; par++) {
@@ -163,11 +197,15 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
             rc = upload_args<S, T>(b, b->d_args[par][rec], args[par][rec]);
             if (!rc && b->tb) rc = upload_args<S, T>(b, b->d_s1[par][rec], s1[par][rec]);
             if (!rc && b->tb) rc = upload_args<S, T>(b, b->d_s2[par][rec], s2[par][rec]);
+            for (int k = 0; k < 4 && !rc && b->qh; k++) rc = upload_args<S, T>(b, b->d_q[k][par][rec], q[k][par][rec]);
         }
         if (!rc && b->tb) rc = upload_args<S, T>(b, b->d_b2[par], b2[par]);
+        if (!rc && b->qh) rc = upload_args<S, T>(b, b->d_b4[par], b4[par]);
     }
     if (!rc) rc = upload_tiles<T>(b, b->d_tiles, tiles);
     if (!rc) rc = upload_tiles<T>(b, b->d_tiles2, tiles2);
+    if (!rc) rc = upload_tiles<T>(b, b->d_tiles4, tiles4);
+    for (int k = 0; k < 4 && !rc; k++) rc = upload_tiles<T>(b, b->d_qtiles[k], qtiles[k]);
     for (int k = 0; k < 3 && !rc; k++) rc = upload_tiles<T>(b, b->d_stiles[k], stiles[k]);
     if (!rc) rc = upload_tiles<T>(b, b->d_wtiles, wtiles);
     if (rc) return rc;
@@ -175,6 +213,8 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
     b->nwtiles = (int)wtiles.size();
     b->ntiles = (int)tiles.size();
     b->ntiles2 = (int)tiles2.size();
+    b->ntiles4 = (int)tiles4.size();
+    for (int k = 0; k < 4; k++) b->nqtiles[k] = (int)qtiles[k].size();
     const int resident = bulk_batch_resident_blocks<S, T>(b->runs[0]->variant);
     b->grid = (b->persistent && resident > 0) ? std::min(resident, b->ntiles) : b->ntiles;
     return 0;
@@ -212,12 +252,38 @@ template <class S, class T> static int launch_pair(mrlbm_batch *b, int par, int 
     return 0;
 }
 
-// graph_steps steps from parity par: pairs (two-step mode) or single steps
+// Four steps of every run from parity par: the four-step bulk on stream A, the four band
+// sub-steps on stream B; the runs end at parity par ^ 1.
+template <class S, class T> static int launch_quad(mrlbm_batch *b, int par, int rec)
+{
+    const int R = (int)b->runs.size();
+    BHIP(hipStreamWaitEvent(b->st, b->evB, 0));
+    BHIP(launch_bulkn_batch<S, T>((const StepArgs<S, T> *)b->d_b4[par], b->d_tiles4, b->ntiles4, b->qh, b->st));
+    BHIP(hipStreamWaitEvent(b->st2, b->evA, 0));
+    for (int k = 0; k < 4; k++)
+        BHIP(launch_side_tiles<S, T>((const StepArgs<S, T> *)b->d_q[k][par][rec], b->d_qtiles[k], b->nqtiles[k],
+                                     b->d_wtiles, b->nwtiles, R, b->wall_levels, rec != 0, b->st2));
+    BHIP(hipEventRecord(b->evB, b->st2));
+    BHIP(hipEventRecord(b->evA, b->st));
+    return 0;
+}
+
+// Steps remaining in a run of n from s, taken by the next pass: four, two or one.
+static int pass_steps(const mrlbm_batch *b, int left)
+{
+    if (b->qh && left >= 4) return 4;
+    return (b->tb && left >= 2) ? 2 : 1;
+}
+
+// n steps from parity par: four-step passes, pairs or single steps
 template <class S, class T> static int launch_steps(mrlbm_batch *b, int par, int rec, int n, int *par_out)
 {
     int rc = 0, s = 0;
     while (s < n && !rc) {
-        if (b->tb && n - s >= 2) {
+        if (pass_steps(b, n - s) == 4) {
+            rc = launch_quad<S, T>(b, par, rec);
+            s += 4;
+        } else if (pass_steps(b, n - s) == 2) {
             rc = launch_pair<S, T>(b, par, rec);
             s += 2;
         } else {
@@ -278,7 +344,7 @@ template <class S, class T> static int step_typed(mrlbm_batch *b, int nsteps, in
             par ^= b->exec_flip;
             s += b->graph_steps;
         } else {
-            const int n = (b->tb && nsteps - s >= 2) ? 2 : 1;
+            const int n = pass_steps(b, nsteps - s);
             const int rc = launch_steps<S, T>(b, par, rec, n, &par);
             if (rc) return rc;
             s += n;
@@ -291,7 +357,8 @@ template <class S, class T> static int step_typed(mrlbm_batch *b, int nsteps, in
 }
 
 // Times every candidate on a saved copy of the states, keeps the fastest: the single-step
-// bulk variants x grid modes, then two steps per pass with the strip heights in `tbs`.
+// bulk variants x grid modes, then two steps per pass with the strip heights in `tbs`, then
+// four steps per pass with the strip heights in `qhs` (the rest of a graph by the best pair).
 template <class S, class T>
 static int autotune_typed(mrlbm_batch *b, int fixed_kernel, int fixed_persistent, int fixed_tb)
 {
@@ -305,8 +372,14 @@ static int autotune_typed(mrlbm_batch *b, int fixed_kernel, int fixed_persistent
     std::vector<int> tbs;
     if (fixed_tb > 0 && b->tb_possible) tbs = {fixed_tb};
     else if (fixed_tb < 0 && b->tb_possible) tbs = b->f32 ? std::vector<int>{6, 10} : std::vector<int>{6, 8};
-    const bool only_tb = fixed_tb > 0 && b->tb_possible;
-    const size_t ncand = (only_tb ? 1 : kernels.size() * modes.size()) + tbs.size();
+    std::vector<int> qhs;
+    if (fixed_tb != 0 && b->q_possible && b->spp != 2) qhs = b->f32 ? std::vector<int>{24, 12} : std::vector<int>{16, 8};
+    // steps_per_pass 4: only the four-step candidates (the single-step variant and pair strip
+    // height of the remainders are the first ones, untimed)
+    const bool only_q = b->spp == 4 && !qhs.empty();
+    const bool only_tb = !only_q && fixed_tb > 0 && b->tb_possible;
+    const bool time_single = !only_q && !only_tb, time_pairs = !only_q;
+    const size_t ncand = (time_single ? kernels.size() * modes.size() : 0) + (time_pairs ? tbs.size() : 0) + qhs.size();
     const bool tune = ncand > 1;
     std::vector<void *> saved;
     const int par0 = b->runs[0]->cur;
@@ -324,13 +397,15 @@ static int autotune_typed(mrlbm_batch *b, int fixed_kernel, int fixed_persistent
     b->tune_kernel.clear();
     b->tune_persistent.clear();
     b->tune_tb.clear();
+    b->tune_quad.clear();
     b->tune_ms.clear();
     BHIP(hipEventRecord(b->evA, b->st));
     BHIP(hipEventRecord(b->evB, b->st));
-    int best_k = kernels[0], best_p = modes[0], best_tb = only_tb ? tbs[0] : 0;
+    int best_k = kernels[0], best_p = modes[0], best_tb = (only_tb || only_q) && !tbs.empty() ? tbs[0] : 0;
+    int best_q = only_q ? qhs[0] : 0;
     double best = 1e300;
     int rc = 0;
-    auto try_candidate = [&](int k, int pm, int tbc) -> int {
+    auto try_candidate = [&](int k, int pm, int tbc, int qhc) -> int {
         std::string why;
         for (mrlbm_handle *h : b->runs) {
             const int r2 = set_bulk_shape(h, k, 0, why);
@@ -338,11 +413,12 @@ static int autotune_typed(mrlbm_batch *b, int fixed_kernel, int fixed_persistent
         }
         b->persistent = pm;
         b->tb = tbc;
+        b->qh = qhc;
         int r3 = build_tables<S, T>(b);
         if (r3 || !tune) return r3;
         const int par = b->runs[0]->cur;
         int pe = par;
-        r3 = launch_steps<S, T>(b, par, 0, 2, &pe);
+        r3 = launch_steps<S, T>(b, par, 0, qhc ? 4 : 2, &pe);
         if (!r3 && hipEventRecord(b->e0, b->st) != hipSuccess) r3 = MRLBM_E_HIP;
         if (!r3) r3 = launch_steps<S, T>(b, pe, 0, b->tune_steps, &pe);
         if (!r3 && hipStreamWaitEvent(b->st, b->evB, 0) != hipSuccess) r3 = MRLBM_E_HIP;
@@ -355,12 +431,14 @@ static int autotune_typed(mrlbm_batch *b, int fixed_kernel, int fixed_persistent
         b->tune_kernel.push_back(k);
         b->tune_persistent.push_back(pm);
         b->tune_tb.push_back(tbc);
+        b->tune_quad.push_back(qhc);
         b->tune_ms.push_back(per);
         if (per < best) {
             best = per;
             best_k = k;
             best_p = pm;
             best_tb = tbc;
+            best_q = qhc;
         }
         // restore the states and parity for the next candidate
         for (size_t r = 0; r < b->runs.size(); r++) {
@@ -371,19 +449,35 @@ static int autotune_typed(mrlbm_batch *b, int fixed_kernel, int fixed_persistent
         }
         return 0;
     };
-    if (!only_tb)
+    if (time_single)
         for (int k : kernels) {
             for (int pm : modes) {
-                rc = try_candidate(k, pm, 0);
+                rc = try_candidate(k, pm, 0, 0);
                 if (rc || !tune) break;
             }
             if (rc || !tune) break;
         }
-    const int single_k = best_k, single_p = best_p; // the remainder steps of a two-step run
-    if (!rc && tune)
+    const int single_k = best_k, single_p = best_p; // the remainder steps of a multi-step run
+    if (!rc && time_pairs && (tune || only_tb))
         for (int tbc : tbs) {
-            rc = try_candidate(single_k, only_tb ? modes[0] : single_p, tbc);
-            if (rc) break;
+            rc = try_candidate(single_k, only_tb ? modes[0] : single_p, tbc, 0);
+            if (rc || !tune) break;
+        }
+    // the pair strip height of the four-step candidates' remainders: the best pair timed
+    int pair_tb = only_q ? best_tb : 0;
+    if (!only_q) {
+        double bp = 1e300;
+        for (size_t i = 0; i < b->tune_ms.size(); i++)
+            if (b->tune_tb[i] > 0 && b->tune_ms[i] < bp) {
+                bp = b->tune_ms[i];
+                pair_tb = b->tune_tb[i];
+            }
+        if (!pair_tb && !tbs.empty()) pair_tb = tbs[0];
+    }
+    if (!rc && (tune || only_q))
+        for (int qhc : qhs) {
+            rc = try_candidate(single_k, single_p, pair_tb, qhc);
+            if (rc || !tune) break;
         }
     for (void *q : saved) (void)hipFree(q);
     if (rc) return rc;
@@ -394,7 +488,8 @@ static int autotune_typed(mrlbm_batch *b, int fixed_kernel, int fixed_persistent
         if (r2) BFAIL(r2, why);
     }
     b->persistent = best_p;
-    b->tb = best_tb;
+    b->tb = best_q ? pair_tb : best_tb;
+    b->qh = best_q;
     return build_tables<S, T>(b);
 }
 
@@ -406,8 +501,13 @@ static void batch_free(mrlbm_batch *b)
             for (void *p : {b->d_args[par][rec], b->d_s1[par][rec], b->d_s2[par][rec]})
                 if (p) (void)hipFree(p);
         if (b->d_b2[par]) (void)hipFree(b->d_b2[par]);
+        if (b->d_b4[par]) (void)hipFree(b->d_b4[par]);
+        for (int k = 0; k < 4; k++)
+            for (int rec = 0; rec < 2; rec++)
+                if (b->d_q[k][par][rec]) (void)hipFree(b->d_q[k][par][rec]);
     }
-    for (int2 *p : {b->d_tiles, b->d_tiles2, b->d_stiles[0], b->d_stiles[1], b->d_stiles[2], b->d_wtiles})
+    for (int2 *p : {b->d_tiles, b->d_tiles2, b->d_tiles4, b->d_stiles[0], b->d_stiles[1], b->d_stiles[2], b->d_wtiles,
+                    b->d_qtiles[0], b->d_qtiles[1], b->d_qtiles[2], b->d_qtiles[3]})
         if (p) (void)hipFree(p);
     if (b->d_facc) (void)hipFree(b->d_facc);
     for (hipEvent_t e : {b->evA, b->evB, b->e0, b->e1})
@@ -453,6 +553,9 @@ extern "C" int mrlbm_batch_create(const mrlbm_params *runs, int32_t nruns, const
     if (opts.time_blocking < -1 || opts.time_blocking > 10 || (opts.time_blocking > 0 && opts.time_blocking % 2))
         return fail(MRLBM_E_ARG, "time_blocking must be -1, 0, 2, 4, 6, 8 or 10");
     if (opts.kernel == 1) return fail(MRLBM_E_ARG, "the batched bulk has no one-node-per-thread kernel");
+    if (opts.steps_per_pass != 0 && opts.steps_per_pass != 2 && opts.steps_per_pass != 4)
+        return fail(MRLBM_E_ARG, "steps_per_pass must be 0, 2 or 4");
+    b->spp = opts.steps_per_pass;
     b->graph_steps = opts.graph_steps;
     b->tune_steps = opts.tune_steps;
     b->f32 = runs[0].precision == MRLBM_FP32_DEV;
@@ -461,15 +564,16 @@ extern "C" int mrlbm_batch_create(const mrlbm_params *runs, int32_t nruns, const
         mrlbm_handle *h = nullptr;
         mrlbm_params pr = runs[r];
         // any strip height: it allocates the scratch state and the band rectangles of the
-        // two-step pass (the batch picks the strip height itself); the batch steps in pairs, so
-        // no four-step scratch state
+        // two-step pass and, unless the batch stays at two steps per pass, the second scratch
+        // state and the rectangles of the four-step pass (the batch picks the strip heights)
         pr.time_blocking = opts.time_blocking == 0 ? 0 : 2;
-        pr.steps_per_pass = 2;
+        pr.steps_per_pass = opts.steps_per_pass == 2 ? 2 : 0;
         const int rc = mrlbm_create(&pr, &h);
         if (rc) return fail(rc, "run " + std::to_string(r) + ": " + mrlbm_last_error(nullptr));
         b->runs.push_back(h);
         b->nodes += (int64_t)h->nx * h->ny;
         if (!h->S2) b->tb_possible = false;
+        if (!h->S3 || !h->ms_H) b->q_possible = false;
         if (!h->d_rec) { // force reduction target of the boundary workgroup
             h->rec_stride = 14 + 2 * h->bt.nbct;
             h->rec_cap = 256;
@@ -561,12 +665,14 @@ extern "C" int mrlbm_batch_get_info(const mrlbm_batch *b, mrlbm_batch_info *info
     info->graph_steps = b->graph_steps;
     info->nodes = b->nodes;
     info->time_blocking = b->tb;
+    info->quad_rows = b->qh;
     info->ntune = (int32_t)std::min<size_t>(16, b->tune_ms.size());
     for (int i = 0; i < info->ntune; i++) {
         info->tune_kernel[i] = b->tune_kernel[i];
         info->tune_persistent[i] = b->tune_persistent[i];
         info->tune_tb[i] = b->tune_tb[i];
         info->tune_ms[i] = b->tune_ms[i];
+        info->tune_quad[i] = b->tune_quad[i];
     }
     return MRLBM_OK;
 }

@@ -149,6 +149,10 @@ hipError_t launch_side_tiles(const StepArgs<S, T> *args, const int2 *stiles, int
                              int nwtiles, int nruns, bool wall_levels, bool records, hipStream_t st);
 template <class S, class T>
 hipError_t launch_bulk2_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int H, hipStream_t st);
+// Four steps per pass for every run of a batch (tiles: (run, four-step block)); strips of H
+// rows (FP32 12 or 24, FP64 8 or 16).
+template <class S, class T>
+hipError_t launch_bulkn_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int H, hipStream_t st);
 // Resident blocks of the batched bulk kernel of a variant on the whole device (0 on error).
 template <class S, class T> int bulk_batch_resident_blocks(int variant);
 

@@ -1202,6 +1202,17 @@ __global__ void __launch_bounds__(256) bulk2_batch_kernel(const StepArgs<S, T> *
     }
 }
 
+// Four steps per pass for every run from the flattened tile table (run, four-step block).
+template <class S, class T, int H>
+__global__ void __launch_bounds__(256, 2) bulkn_batch_kernel(const StepArgs<S, T> *args, const int2 *tiles, int ntiles)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        const StepArgs<S, T> a = args[tl.x];
+        bulk_march_n<S, T, 4, H, false>(a, tl.y);
+    }
+}
+
 template <class S, class T> __global__ void __launch_bounds__(256) bulk_simple_kernel(StepArgs<S, T> a)
 {
     bulk_simple(a, blockIdx.x);
@@ -1563,6 +1574,21 @@ hipError_t launch_bulk2_batch(const StepArgs<S, T> *args, const int2 *tiles, int
     return hipGetLastError();
 }
 template <class S, class T>
+hipError_t launch_bulkn_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int H, hipStream_t st)
+{
+    const dim3 g(ntiles), b(256);
+    if constexpr (sizeof(S) == 4) {
+        if (H == 24) hipLaunchKernelGGL((bulkn_batch_kernel<S, T, 24>), g, b, 0, st, args, tiles, ntiles);
+        else if (H == 12) hipLaunchKernelGGL((bulkn_batch_kernel<S, T, 12>), g, b, 0, st, args, tiles, ntiles);
+        else return hipErrorInvalidValue;
+    } else {
+        if (H == 16) hipLaunchKernelGGL((bulkn_batch_kernel<S, T, 16>), g, b, 0, st, args, tiles, ntiles);
+        else if (H == 8) hipLaunchKernelGGL((bulkn_batch_kernel<S, T, 8>), g, b, 0, st, args, tiles, ntiles);
+        else return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template <class S, class T>
 hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0, hipEvent_t t1)
 {
     const dim3 g(nblocks), b(256);
@@ -1746,6 +1772,7 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_side_tiles<S, T>(const StepArgs<S, T> *, const int2 *, int, const int2 *, int, int, bool, \
                                                 bool, hipStream_t);                                  \
     template hipError_t launch_bulk2_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, hipStream_t); \
+    template hipError_t launch_bulkn_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, hipStream_t); \
     template int bulk_batch_resident_blocks<S, T>(int);                                          \
     template hipError_t launch_inlet_fold<S, T>(const StepArgs<S, T> &, hipStream_t);            \
     template hipError_t launch_pair_probes<S, T>(const StepArgs<S, T> &, const S *, int, hipStream_t); \

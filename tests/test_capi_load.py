@@ -21,7 +21,7 @@ def test_library_exports_every_symbol():
     L = capi.load()
     for name in declared_symbols():
         assert hasattr(L, name), name
-    assert L.mrlbm_abi_version() == 1
+    assert L.mrlbm_abi_version() == 2
 
 
 def test_default_params_match_reference_input():

@@ -1,6 +1,7 @@
 """Batched runs (mrlbm_batch_*): runs of uneven size advanced in shared launches and
-captured graphs give the bits of the same runs stepped alone, for every bulk variant and
-grid mode, and the per-run force sums equal the single-run series summed in order."""
+captured graphs give the bits of the same runs stepped alone, for every bulk variant, grid
+mode and one, two or four steps per pass, and the per-run force sums equal the single-run
+series summed in order."""
 import numpy as np
 import pytest
 
@@ -37,23 +38,29 @@ def single(p, run, n1, n2):
 
 
 @pytest.mark.parametrize("precision", [capi.MRLBM_FP32_DEV, capi.MRLBM_FP64])
-@pytest.mark.parametrize("kernel,persistent,time_blocking", [(0, -1, -1), (8, 0, 0), (304, 1, 0), (2, 1, 0),
-                                                             (304, 0, 4), (8, 0, 2), (16, 1, 6)])
-def test_batch_matches_single_runs(precision, kernel, persistent, time_blocking):
+@pytest.mark.parametrize("kernel,persistent,time_blocking,spp", [
+    (0, -1, -1, 0), (8, 0, 0, 0), (304, 1, 0, 0), (2, 1, 0, 0), (304, 0, 4, 2), (8, 0, 2, 2), (16, 1, 6, 2),
+    (304, 0, 6, 4), (8, 1, -1, 4)])
+def test_batch_matches_single_runs(precision, kernel, persistent, time_blocking, spp):
     ps = runs(precision)
     o = capi.default_batch_opts()
     o.kernel, o.persistent, o.graph_steps, o.time_blocking = kernel, persistent, 22, time_blocking
-    # 22-step graphs: 41 = one graph + direct steps (a pair and a single step in two-step
-    # mode), so the second call starts at the other parity; 60 = two graphs + 16 direct
+    o.steps_per_pass = spp
+    # 22-step graphs: 41 = one graph + direct steps (four-step passes, a pair, a single step),
+    # so the second call starts at the other parity; 60 = two graphs + 16 direct
     n1, n2 = 41, 60
     with capi.Batch(ps, o) as b:
         info = b.info
         print(f"[gpu] batch prec {precision} kernel {info.kernel} persistent {info.persistent} time_blocking "
-              f"{info.time_blocking} grid {info.grid} tiles {info.ntiles} tuned "
-              f"{[(info.tune_kernel[i], info.tune_persistent[i], info.tune_tb[i], round(info.tune_ms[i], 4)) for i in range(info.ntune)]}",
+              f"{info.time_blocking} four-step rows {info.quad_rows} grid {info.grid} tiles {info.ntiles} tuned "
+              f"{[(info.tune_kernel[i], info.tune_persistent[i], info.tune_tb[i], info.tune_quad[i], round(info.tune_ms[i], 4)) for i in range(info.ntune)]}",
               flush=True)
-        if time_blocking > 0:
+        if time_blocking > 0 and spp != 4:
             assert info.time_blocking == time_blocking
+        if spp == 4:
+            assert info.quad_rows in ((24, 12) if precision == capi.MRLBM_FP32_DEV else (16, 8))
+        if spp == 2 or time_blocking == 0:
+            assert info.quad_rows == 0
         if kernel == 0:
             assert info.ntune >= 2
             best = min(range(info.ntune), key=lambda i: info.tune_ms[i])
