@@ -47,8 +47,8 @@ HBM_PEAK_GBS = 8000.0
 # (--steps-per-pass 2; FP32 10 rows, FP64 12), with the profiles holding the summaries and
 # the commands
 TRAFFIC = {
-    4: ({"f32": 1_043_207_473, "f64": 2_121_719_764}, "profiles/bulkn_kernel_8192x2048_hbm_bytes.md"),
-    2: ({"f32": 949_550_578, "f64": 2_289_678_504}, "profiles/bulk2_kernel_8192x2048_hbm_bytes.md"),
+    4: ({"f32": 1_043_207_473, "f64": 2_121_719_764}, "profiles/bulkn_kernel_8192x2048_hbm_bytes.md", (24, 16)),
+    2: ({"f32": 949_550_578, "f64": 2_289_678_504}, "profiles/bulk2_kernel_8192x2048_hbm_bytes.md", (10, 12)),
 }
 NX, NY = 8192, 2048
 # HIP event pairs around every 4th bulk launch on long runs: the per-launch mean without the
@@ -63,7 +63,7 @@ REF_RATIO_8T, REF_RATIO_1T = (3.52, 3.73), (1.29, 2.15)
 UMAX, RE = 0.05, 200.0
 
 
-def make_lattice(precision, nx=NX, ny=NY, strip_rows=0, kernel=0, time_blocking=-1, steps_per_pass=0):
+def make_lattice(precision, nx=NX, ny=NY, strip_rows=0, kernel=0, time_blocking=-1, steps_per_pass=0, quad_rows=0):
     from mrlbm_2d_amd import capi
     p = capi.default_params()
     p.nx, p.ny = nx, ny
@@ -77,6 +77,7 @@ def make_lattice(precision, nx=NX, ny=NY, strip_rows=0, kernel=0, time_blocking=
     p.kernel = kernel
     p.time_blocking = time_blocking
     p.steps_per_pass = steps_per_pass
+    p.quad_rows = quad_rows
     lat = capi.Lattice(p)
     rng = np.random.default_rng(3)
     ux = rng.uniform(-1e-4, 1e-4, (nx, ny))
@@ -139,7 +140,7 @@ def run_form(precision, steps, warmup, gate_steps=0, gate_window=0, **kw):
                 launch_max_ms=t.bulk_max_ms, timed_launches=t.timed_launches, bulk_launches=t.bulk_launches,
                 steps_per_launch=t.bulk_steps / max(t.bulk_launches, 1), timed_steps=t.timed_steps,
                 bytes_per_step=bytes_per_step, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
-                first_timed_step=it), out
+                first_timed_step=it, pair_rows=info.pair_rows, quad_rows=info.quad_rows), out
 
 
 def roofline(r, kernel, traffic, source):
@@ -415,13 +416,15 @@ def main():
         spp = 1
         k32, k64 = "bulk_kernel<float,float,false,304>", "bulk_kernel<double,double,false,8>"
     elif spp == 4:
-        k32, k64 = "bulkn_kernel<float,float,4,24>", "bulkn_kernel<double,double,4,16>"
+        k32 = f"bulkn_kernel<float,float,4,{r32['quad_rows']},false>"
+        k64 = f"bulkn_kernel<double,double,4,{r64['quad_rows']},false>"
     else:
-        h32, h64 = (10, 12) if tb == -1 else (tb, tb)
-        k32, k64 = f"bulk2_kernel<float,float,{h32},false>", f"bulk2_kernel<double,double,{h64},false>"
+        k32 = f"bulk2_kernel<float,float,{r32['pair_rows']},false>"
+        k64 = f"bulk2_kernel<double,double,{r64['pair_rows']},false>"
     # measured traffic applies to the profiled launch shapes only
-    traffic, source = TRAFFIC.get(spp, ({"f32": None, "f64": None}, None))
-    if args.strip_rows != 0 or tb not in (-1,):
+    traffic, source, shape = TRAFFIC.get(spp, ({"f32": None, "f64": None}, None, None))
+    rows = (r32["quad_rows"], r64["quad_rows"]) if spp == 4 else (r32["pair_rows"], r64["pair_rows"])
+    if args.strip_rows != 0 or tb not in (-1,) or rows != shape:
         traffic = {"f32": None, "f64": None}
     out = {
         # the headline metric: MLUPS of the FP32 deviation form (value), its % of HBM peak

@@ -67,10 +67,12 @@ typedef struct mrlbm_params {
                                      auto: 10 rows FP32, 12 rows FP64.  Same bits either way */
     int32_t steps_per_pass;       /* with time_blocking on: 0 (default) or 4: four steps per bulk pass
                                      over the plain interior where four consecutive steps allow it
-                                     (grids of at least 32 x 32; no u-statistics sample, probe record
-                                     or plot step among them), else pairs; 2: pairs only.  Same bits
-                                     either way */
-    int32_t reserved[3];
+                                     (grids of at least 32 x 32; no plot step among them), else pairs;
+                                     2: pairs only.  Same bits either way */
+    int32_t quad_rows;            /* four-step strip height: 0 (default) auto (20 rows; FP32 12, FP64 8
+                                     below 4096 waves), else FP32 12, 16, 20, 24, 28 or 32, FP64 8,
+                                     16, 20 or 24.  Same bits either way */
+    int32_t reserved[2];
 } mrlbm_params;
 
 /* What one call to mrlbm_step records (2d_mrlbm.f90:355-502). */
@@ -93,6 +95,8 @@ typedef struct mrlbm_info {
     int32_t box[4];               /* 0-based x0, y0, width, height of the cylinder box */
     int32_t n_stencil;            /* unique stencil nodes of the boundary workgroup */
     int32_t levels;               /* dependency levels of the in-place velocity interpolation */
+    int32_t pair_rows;            /* strip height of two-step passes, 0 = none */
+    int32_t quad_rows;            /* strip height of four-step passes, 0 = none */
 } mrlbm_info;
 
 typedef struct mrlbm_handle mrlbm_handle;

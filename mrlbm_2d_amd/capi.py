@@ -49,7 +49,7 @@ class Params(C.Structure):
         ("incomp_bc", C.c_int32), ("incomp_bulk", C.c_int32),
         ("precision", C.c_int32), ("label_policy", C.c_int32), ("strip_rows", C.c_int32),
         ("kernel", C.c_int32), ("block_order", C.c_int32), ("time_blocking", C.c_int32),
-        ("steps_per_pass", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("steps_per_pass", C.c_int32), ("quad_rows", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -75,7 +75,7 @@ class Info(C.Structure):
         ("r_cyl", C.c_double), ("nu", C.c_double), ("omega", C.c_double),
         ("x_c", C.c_double), ("y_c", C.c_double), ("r_c", C.c_double),
         ("bytes_per_step", C.c_int64), ("box", C.c_int32 * 4), ("n_stencil", C.c_int32),
-        ("levels", C.c_int32),
+        ("levels", C.c_int32), ("pair_rows", C.c_int32), ("quad_rows", C.c_int32),
     ]
 
 

@@ -373,7 +373,8 @@ static int autotune_typed(mrlbm_batch *b, int fixed_kernel, int fixed_persistent
     if (fixed_tb > 0 && b->tb_possible) tbs = {fixed_tb};
     else if (fixed_tb < 0 && b->tb_possible) tbs = b->f32 ? std::vector<int>{6, 10} : std::vector<int>{6, 8};
     std::vector<int> qhs;
-    if (fixed_tb != 0 && b->q_possible && b->spp != 2) qhs = b->f32 ? std::vector<int>{24, 12} : std::vector<int>{16, 8};
+    if (fixed_tb != 0 && b->q_possible && b->spp != 2)
+        qhs = b->f32 ? std::vector<int>{20, 24, 12} : std::vector<int>{20, 16, 8};
     // steps_per_pass 4: only the four-step candidates (the single-step variant and pair strip
     // height of the remainders are the first ones, untimed)
     const bool only_q = b->spp == 4 && !qhs.empty();

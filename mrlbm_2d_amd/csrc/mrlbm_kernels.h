@@ -124,6 +124,8 @@ hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t
 template <class S, class T>
 hipError_t launch_bulkn(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0 = nullptr,
                         hipEvent_t t1 = nullptr);
+// Whether the four-step kernels are built for strips of H rows in this precision.
+template <class S> bool bulkn_rows_ok(int H);
 // Workgroups of the ring's inlet and outlet columns when they go by column tiles (64 rows
 // each, staged in LDS): grids at least 8 wide stepping from moments; 0 otherwise (the
 // columns are then ordinary ring items).

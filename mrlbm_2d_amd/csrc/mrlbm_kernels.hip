@@ -15,6 +15,7 @@
 #include "mrlbm_device.h"
 
 #include <algorithm>
+#include <type_traits>
 
 #include <hip/hip_ext.h>
 
@@ -1577,16 +1578,11 @@ template <class S, class T>
 hipError_t launch_bulkn_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int H, hipStream_t st)
 {
     const dim3 g(ntiles), b(256);
-    if constexpr (sizeof(S) == 4) {
-        if (H == 24) hipLaunchKernelGGL((bulkn_batch_kernel<S, T, 24>), g, b, 0, st, args, tiles, ntiles);
-        else if (H == 12) hipLaunchKernelGGL((bulkn_batch_kernel<S, T, 12>), g, b, 0, st, args, tiles, ntiles);
-        else return hipErrorInvalidValue;
-    } else {
-        if (H == 16) hipLaunchKernelGGL((bulkn_batch_kernel<S, T, 16>), g, b, 0, st, args, tiles, ntiles);
-        else if (H == 8) hipLaunchKernelGGL((bulkn_batch_kernel<S, T, 8>), g, b, 0, st, args, tiles, ntiles);
-        else return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    const bool ok = with_bulkn_rows<S>(H, [&](auto hh) {
+        constexpr int HH = decltype(hh)::value;
+        hipLaunchKernelGGL((bulkn_batch_kernel<S, T, HH>), g, b, 0, st, args, tiles, ntiles);
+    });
+    return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 template <class S, class T>
 hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0, hipEvent_t t1)
@@ -1612,31 +1608,47 @@ hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t
 #undef MRLBM_BULK2
     return hipGetLastError();
 }
+// Four-step strip heights built: FP32 12..32 (multiples of 4), FP64 8, 16, 20, 24 (at 12 rows
+// the compiler moves the FP64 march's carried rows to scratch memory).
+template <class S> bool bulkn_rows_ok(int H)
+{
+    if constexpr (sizeof(S) == 4) return H == 12 || H == 16 || H == 20 || H == 24 || H == 28 || H == 32;
+    else return H == 8 || H == 16 || H == 20 || H == 24;
+}
+// calls f with std::integral_constant<int, H> for a built strip height H (false otherwise)
+template <class S, class F> static bool with_bulkn_rows(int H, F &&f)
+{
+    if constexpr (sizeof(S) == 4) {
+        switch (H) {
+        case 12: f(std::integral_constant<int, 12>{}); return true;
+        case 16: f(std::integral_constant<int, 16>{}); return true;
+        case 20: f(std::integral_constant<int, 20>{}); return true;
+        case 24: f(std::integral_constant<int, 24>{}); return true;
+        case 28: f(std::integral_constant<int, 28>{}); return true;
+        case 32: f(std::integral_constant<int, 32>{}); return true;
+        }
+    } else {
+        switch (H) {
+        case 8: f(std::integral_constant<int, 8>{}); return true;
+        case 16: f(std::integral_constant<int, 16>{}); return true;
+        case 20: f(std::integral_constant<int, 20>{}); return true;
+        case 24: f(std::integral_constant<int, 24>{}); return true;
+        }
+    }
+    return false;
+}
 template <class S, class T>
 hipError_t launch_bulkn(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0, hipEvent_t t1)
 {
     const dim3 g(nblocks), b(256);
-    // strip heights: FP32 24 (12 on small grids), FP64 16 (8); the recording variant when a
-    // level is sampled
+    // the recording variant when a level is sampled
     const bool rec = a.qsample != 0;
-    if constexpr (sizeof(S) == 4) {
-        if (H == 24) {
-            if (rec) launch_timed(bulkn_kernel<S, T, 4, 24, true>, g, b, st, t0, t1, a);
-            else launch_timed(bulkn_kernel<S, T, 4, 24>, g, b, st, t0, t1, a);
-        } else if (H == 12) {
-            if (rec) launch_timed(bulkn_kernel<S, T, 4, 12, true>, g, b, st, t0, t1, a);
-            else launch_timed(bulkn_kernel<S, T, 4, 12>, g, b, st, t0, t1, a);
-        } else return hipErrorInvalidValue;
-    } else {
-        if (H == 16) {
-            if (rec) launch_timed(bulkn_kernel<S, T, 4, 16, true>, g, b, st, t0, t1, a);
-            else launch_timed(bulkn_kernel<S, T, 4, 16>, g, b, st, t0, t1, a);
-        } else if (H == 8) {
-            if (rec) launch_timed(bulkn_kernel<S, T, 4, 8, true>, g, b, st, t0, t1, a);
-            else launch_timed(bulkn_kernel<S, T, 4, 8>, g, b, st, t0, t1, a);
-        } else return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    const bool ok = with_bulkn_rows<S>(H, [&](auto hh) {
+        constexpr int HH = decltype(hh)::value;
+        if (rec) launch_timed(bulkn_kernel<S, T, 4, HH, true>, g, b, st, t0, t1, a);
+        else launch_timed(bulkn_kernel<S, T, 4, HH>, g, b, st, t0, t1, a);
+    });
+    return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st, bool stencil)
 {
@@ -1773,6 +1785,7 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
                                                 bool, hipStream_t);                                  \
     template hipError_t launch_bulk2_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, hipStream_t); \
     template hipError_t launch_bulkn_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, hipStream_t); \
+    template bool bulkn_rows_ok<S>(int);                                                         \
     template int bulk_batch_resident_blocks<S, T>(int);                                          \
     template hipError_t launch_inlet_fold<S, T>(const StepArgs<S, T> &, hipStream_t);            \
     template hipError_t launch_pair_probes<S, T>(const StepArgs<S, T> &, const S *, int, hipStream_t); \

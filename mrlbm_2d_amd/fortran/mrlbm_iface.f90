@@ -19,8 +19,8 @@ module mrlbm_iface
         integer(c_int32_t) :: vel_interp, mom_interp
         integer(c_int32_t) :: incomp_bc, incomp_bulk
         integer(c_int32_t) :: precision, label_policy, strip_rows, kernel, block_order
-        integer(c_int32_t) :: time_blocking, steps_per_pass
-        integer(c_int32_t) :: reserved(3)
+        integer(c_int32_t) :: time_blocking, steps_per_pass, quad_rows
+        integer(c_int32_t) :: reserved(2)
     end type
 
     type, bind(C) :: mrlbm_step_opts
@@ -34,7 +34,7 @@ module mrlbm_iface
         integer(c_int32_t) :: nx, ny, nbct, precision
         real(c_double) :: r_cyl, nu, omega, x_c, y_c, r_c
         integer(c_int64_t) :: bytes_per_step
-        integer(c_int32_t) :: box(4), n_stencil, levels
+        integer(c_int32_t) :: box(4), n_stencil, levels, pair_rows, quad_rows
     end type
 
     type, bind(C) :: mrlbm_adjoint_opts

@@ -23,7 +23,7 @@ def test_roofline_fields_and_arithmetic():
     r = dict(achieved_gbs=7000.0, frac=7000.0 / bench.HBM_PEAK_GBS, steps_per_launch=2.0,
              bytes_per_step=805_306_368, launch_ms=0.23, launch_median_ms=0.229, launch_min_ms=0.22,
              launch_max_ms=0.25, timed_launches=10, bulk_launches=10, timed_steps=20)
-    traffic, source = bench.TRAFFIC[2]
+    traffic, source, _ = bench.TRAFFIC[2]
     out = bench.roofline(r, "bulk2_kernel<float,float,10,false>", traffic["f32"], source)
     assert out["bound"] == "hbm" and out["unit"] == "GB/s" and out["peak"] == 8000.0
     assert out["frac"] == pytest.approx(0.875)
@@ -32,7 +32,7 @@ def test_roofline_fields_and_arithmetic():
     assert out["traffic_source"].startswith(source)
     assert out["traffic_gbs"] == pytest.approx(traffic["f32"] / 0.23e-3 / 1e9, rel=1e-4)
     assert out["traffic_frac"] == pytest.approx(out["traffic_gbs"] / 8000.0, rel=1e-3)
-    for spp, (tr, src) in bench.TRAFFIC.items():
+    for spp, (tr, src, _) in bench.TRAFFIC.items():
         assert os.path.exists(os.path.join(REPO, src)), src
         assert tr["f32"] > 0 and tr["f64"] > 0, spp
     none = bench.roofline(r, "k", None, None)

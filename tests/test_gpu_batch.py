@@ -58,7 +58,7 @@ def test_batch_matches_single_runs(precision, kernel, persistent, time_blocking,
         if time_blocking > 0 and spp != 4:
             assert info.time_blocking == time_blocking
         if spp == 4:
-            assert info.quad_rows in ((24, 12) if precision == capi.MRLBM_FP32_DEV else (16, 8))
+            assert info.quad_rows in ((20, 24, 12) if precision == capi.MRLBM_FP32_DEV else (20, 16, 8))
         if spp == 2 or time_blocking == 0:
             assert info.quad_rows == 0
         if kernel == 0:
