@@ -43,11 +43,11 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel at 8192 x 2048, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1):
-# the four-step bulk pass (default; FP32 strips of 24 rows, FP64 of 16) and the two-step one
+# the four-step bulk pass (default; strips of 20 rows in both forms) and the two-step one
 # (--steps-per-pass 2; FP32 10 rows, FP64 12), with the profiles holding the summaries and
-# the commands
+# the commands; the strip heights each was measured at
 TRAFFIC = {
-    4: ({"f32": 1_043_207_473, "f64": 2_121_719_764}, "profiles/bulkn_kernel_8192x2048_hbm_bytes.md", (24, 16)),
+    4: ({"f32": 1_070_583_778, "f64": 2_017_994_926}, "profiles/bulkn_kernel_8192x2048_hbm_bytes.md", (20, 20)),
     2: ({"f32": 949_550_578, "f64": 2_289_678_504}, "profiles/bulk2_kernel_8192x2048_hbm_bytes.md", (10, 12)),
 }
 NX, NY = 8192, 2048

@@ -20,7 +20,7 @@ def summarize(src: str, dst: str) -> None:
     with open(src) as fh:
         for r in csv.DictReader(fh):
             d = float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
-            a = agg[(r["Kernel_Name"], r.get("Grid_Size", ""))]
+            a = agg[(r["Kernel_Name"], r.get("Grid_Size") or r.get("Grid_Size_X", ""))]
             a[0] += 1
             a[1] += d
             a[2] = min(a[2], d)
