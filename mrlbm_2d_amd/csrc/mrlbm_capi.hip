@@ -173,6 +173,14 @@ template <class S, class T> static int create_typed(mrlbm_handle *h)
         rc |= upload(h, &p, t.idx); b.idx = (const int32_t *)p;
         rc |= upload(h, &p, t.thetas); b.thetas = (const double *)p;
         rc |= upload(h, &p, t.boxcls); h->d_box = (uint8_t *)p;
+        WallGroups wg;
+        if (build_wall_groups(t, WALL_GROUP, WALL_GROUP_MAXU, wg)) {
+            rc |= upload(h, &p, wg.off); b.wg_off = (const int32_t *)p;
+            rc |= upload(h, &p, wg.list); b.wg_list = (const int32_t *)p;
+            rc |= upload(h, &p, wg.node); b.wg_node = (const int32_t *)p;
+            rc |= upload(h, &p, wg.slot); b.wl_slot = (const int16_t *)p;
+            b.nwg = (int)wg.off.size() - 1;
+        }
         if (rc) return rc;
         label_masks(h->p.label_policy, b.incs, b.outs);
         for (int l = 0; l < 16; l++) { // rest-population part of the momentum exchange (FP32 form)
@@ -709,6 +717,7 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         a.inlet_out = window ? h->d_inlet[iter & 1] : nullptr;
         a.inlet_prev = (h->prev_inlet_valid && h->has_b) ? h->d_inlet[h->prev_inlet_buf] : nullptr;
         a.inlet_prev_c = h->prev_inlet_c;
+        a.wall_groups = h->bd.nwg; // the wall nodes as wall groups inside the side launch
         return a;
     };
     // Side work of one iteration on stream B: the ring, band and stencil-node kernel, then the
@@ -717,7 +726,7 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
     // kernel; every launch beside the bulk costs it time, and the fork and join with their
     // extra launch cost more than the overlap gained (8192 x 2048: 2.2% FP32, 1% FP64).
     auto side_work = [&](const StepArgs<S, T> &a) -> hipError_t {
-        hipError_t e = launch_side<S, T>(a, h->st2, true);
+        hipError_t e = launch_side<S, T>(a, h->st2, a.wall_groups == 0);
         if (e != hipSuccess) return e;
         if (h->has_b) return launch_wall<S, T>(a, h->st2);
         if (h->prev_inlet_valid) {

@@ -140,22 +140,56 @@ template <class S, class T> __device__ void gather_interior(const StepArgs<S, T>
     }
 }
 
-// Pre-collision rho, u and rotated Mxx', Myy' of stencil node u (moments m as F::moments
-// gives them) into the wall's stencil table (2d_mrlbm.f90:840-869 for the rotation).
+// The five stencil values of stencil node u (moments m as F::moments gives them): rho, ux,
+// uy and, for the rotated wall BC with moment interpolation, the rotated Mxx', Myy' (else 0;
+// 2d_mrlbm.f90:840-869 for the rotation).
+template <class T> __device__ __forceinline__ void stencil_values(const BoundaryDev &b, int u, const T m[6], T v[5])
+{
+    v[0] = m[0];
+    v[1] = m[1];
+    v[2] = m[2];
+    v[3] = v[4] = (T)0.0;
+    if (b.rotated && b.mom_interp) {
+        const T c2 = (T)b.urot[3 * u], s2 = (T)b.urot[3 * u + 1], sc = (T)b.urot[3 * u + 2];
+        v[3] = ((m[3] * c2) + (m[4] * s2)) + (m[5] * sc);
+        v[4] = ((m[3] * s2) + (m[4] * c2)) - (m[5] * sc);
+    }
+}
+
+// Pre-collision values of stencil node u into the wall's stencil table.
 template <class S, class T> __device__ __forceinline__ void store_stencil_moments(const StepArgs<S, T> &a, int u, const T m[6])
 {
     const BoundaryDev &b = a.b;
     T *U = (T *)b.U;
     const int nU = b.nU;
-    U[u] = m[0];
-    U[nU + u] = m[1];
-    U[2 * nU + u] = m[2];
+    T v[5];
+    stencil_values(b, u, m, v);
+    U[u] = v[0];
+    U[nU + u] = v[1];
+    U[2 * nU + u] = v[2];
     if (b.rotated && b.mom_interp) {
-        const T c2 = (T)b.urot[3 * u], s2 = (T)b.urot[3 * u + 1], sc = (T)b.urot[3 * u + 2];
-        U[3 * nU + u] = ((m[3] * c2) + (m[4] * s2)) + (m[5] * sc);
-        U[4 * nU + u] = ((m[3] * s2) + (m[4] * c2)) - (m[5] * sc);
+        U[3 * nU + u] = v[3];
+        U[4 * nU + u] = v[4];
     }
 }
+
+// Stencil values read by boundary node k: plane p (0 rho, 1 ux, 2 uy, 3 Mxx', 4 Myy') of its
+// reference c (0..11 the corners uc[12k + c], 12 the node itself), from the stencil table
+// in memory (UGlobal) or from a wall group's LDS slots (ULocal).
+template <class T> struct UGlobal {
+    const T *U;
+    int nU;
+    const int32_t *uc, *uself;
+    __device__ __forceinline__ T operator()(int k, int p, int c) const
+    {
+        return U[p * nU + (c < 12 ? uc[12 * k + c] : uself[k])];
+    }
+};
+template <class T> struct ULocal {
+    const T (*sU)[WALL_GROUP_MAXU];
+    const int16_t *slot;
+    __device__ __forceinline__ T operator()(int k, int p, int c) const { return sU[p][slot[13 * k + c]]; }
+};
 
 // Dirichlet values and the regularized edge BC, in the reference's order
 // outlet -> inlet -> top -> bottom with the last writer winning (2d_mrlbm.f90:247-351).

@@ -315,4 +315,38 @@ std::string build_tables(const Geometry &g, int pitch, BoundaryTables &t)
     return "";
 }
 
+bool build_wall_groups(const BoundaryTables &t, int group, int maxu, WallGroups &w)
+{
+    w = WallGroups{};
+    const int nb = t.nbct;
+    if (nb == 0 || t.nlevels > 1) return false;
+    w.node.assign(t.idx.begin(), t.idx.end()); // theta order: neighbours share stencil nodes
+    w.slot.assign((size_t)13 * nb, 0);
+    w.off.push_back(0);
+    for (int g0 = 0; g0 < nb; g0 += group) {
+        std::vector<int32_t> us; // the group's stencil nodes, sorted
+        for (int q = g0; q < std::min(nb, g0 + group); q++) {
+            const int k = w.node[q];
+            for (int c = 0; c < 12; c++) us.push_back(t.uc[12 * k + c]);
+            us.push_back(t.uself[k]);
+        }
+        std::sort(us.begin(), us.end());
+        us.erase(std::unique(us.begin(), us.end()), us.end());
+        if ((int)us.size() > maxu) {
+            w = WallGroups{};
+            return false;
+        }
+        for (int q = g0; q < std::min(nb, g0 + group); q++) {
+            const int k = w.node[q];
+            for (int c = 0; c < 13; c++) {
+                const int u = c < 12 ? t.uc[12 * k + c] : t.uself[k];
+                w.slot[13 * k + c] = (int16_t)(std::lower_bound(us.begin(), us.end(), u) - us.begin());
+            }
+        }
+        w.list.insert(w.list.end(), us.begin(), us.end());
+        w.off.push_back((int32_t)w.list.size());
+    }
+    return true;
+}
+
 } // namespace mrlbm

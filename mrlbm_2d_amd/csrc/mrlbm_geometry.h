@@ -78,6 +78,17 @@ enum {
 
 std::string build_tables(const Geometry &g, int pitch, BoundaryTables &t);
 
+// Boundary nodes in groups of `group` (theta order) with the stencil nodes each group reads:
+// off (groups + 1 offsets into list), list (stencil node indices), node (boundary node of each
+// group position), slot (13 per boundary node: group-local slot of its uc corners 0..11 and of
+// itself).  False (and empty) for a wall of several dependency levels or when a group would
+// read more than maxu stencil nodes.
+struct WallGroups {
+    std::vector<int32_t> off, list, node;
+    std::vector<int16_t> slot;
+};
+bool build_wall_groups(const BoundaryTables &t, int group, int maxu, WallGroups &w);
+
 // Incoming / outgoing masks per label (2d_mrlbm.f90:2005-2057) as 9-bit masks.
 void label_masks(int label_policy, uint16_t incs[16], uint16_t outs[16]);
 

@@ -19,6 +19,13 @@ struct BoundaryDev {
     uint16_t incs[16], outs[16];
     double wfx[16], wfy[16]; // rest-population force per label (FP32 form only)
     int mass_bc, rotated, vel_interp, mom_interp;
+    // wall groups (walls of one dependency level): group g takes the boundary nodes
+    // wg_node[g*WALL_GROUP ...] (theta order) and the stencil nodes wg_list[wg_off[g] ..
+    // wg_off[g+1]) it reads; wl_slot[13*k + c] is the group-local slot of boundary node k's
+    // stencil reference c (0..11 the uc corners, 12 the node itself)
+    int nwg;
+    const int32_t *wg_off, *wg_list, *wg_node;
+    const int16_t *wl_slot;
     void *U;      // 5*nU values of the step type: rho, ux, uy, Mxx', Myy' (pre-collision)
     void *ub;     // 2*nbct interpolated velocities
     double *pcy;  // nbct surface pressure, list order
@@ -62,11 +69,17 @@ template <class S, class T> struct StepArgs {
     double inlet_prev_c;       // its mean counter
     double *rho_inf_mean;
     double *facc;              // batched runs: running sums of Fd, Fl, Fl^2 and the step count
+    int wall_groups;           // side kernel: the first wall_groups blocks are wall groups (stencil
+                               // moments in LDS, then the wall nodes); 0: the stencil nodes are
+                               // side items and the wall nodes a launch of their own
     int nrect;                 // side kernel: band rectangles x0, y0, w, h
     int rect[6][4];
     const T *chi;              // Brinkman solid fraction (per-node bulk kernel only), or null
     BoundaryDev b;
 };
+
+// Boundary nodes per wall group, and the most stencil nodes a group may read (LDS slots).
+constexpr int WALL_GROUP = 32, WALL_GROUP_MAXU = 256;
 
 // Host launchers (mrlbm.hip).
 template <class S, class T>

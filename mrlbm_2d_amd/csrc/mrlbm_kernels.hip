@@ -805,11 +805,21 @@ template <class S, class T> __device__ void ring_column_tile(const StepArgs<S, T
     }
 }
 
-// Side workgroups of one step (single waves): the ring columns as tiles when the column path
-// applies (ring_column_tiles), then 64 items each of the ring rows (or the whole ring), the
-// band rectangles and the stencil nodes.
-template <class S, class T> __device__ void side_block(const StepArgs<S, T> &a, int blk, int nband)
+template <class S, class T> __device__ void wall_group(const StepArgs<S, T> &a, int g);
+
+// Side workgroups of one step (single waves): the wall groups (WG, a.wall_groups of them), the
+// ring columns as tiles when the column path applies (ring_column_tiles), then 64 items each
+// of the ring rows (or the whole ring), the band rectangles and (without wall groups) the
+// stencil nodes.
+template <class S, class T, bool WG = false> __device__ void side_block(const StepArgs<S, T> &a, int blk, int nband)
 {
+    if constexpr (WG) {
+        if (blk < a.wall_groups) {
+            wall_group(a, blk);
+            return;
+        }
+        blk -= a.wall_groups;
+    }
     const int nct = ring_column_tiles(a.nx, a.ny, a.pops != nullptr);
     if (blk < nct) {
         ring_column_tile(a, blk & 1, 1 + (blk >> 1) * 64);
@@ -819,9 +829,9 @@ template <class S, class T> __device__ void side_block(const StepArgs<S, T> &a, 
     side_body(a, (long long)(blk - nct) * 64 + threadIdx.x, nring, nband);
 }
 
-template <class S, class T> __global__ void __launch_bounds__(64) side_kernel(StepArgs<S, T> a, int nband)
+template <class S, class T, bool WG> __global__ void __launch_bounds__(64) side_kernel(StepArgs<S, T> a, int nband)
 {
-    side_block(a, blockIdx.x, nband);
+    side_block<S, T, WG>(a, blockIdx.x, nband);
 }
 
 // Bulk, one node per thread: the first step after mrlbm_set_populations (populations in
@@ -879,12 +889,13 @@ template <class S, class T> __global__ void __launch_bounds__(256) stencil_momen
 // boundary velocity is interpolated from another boundary node's): the node interpolates its
 // own wall velocity, its stencil loads issued together with the gather so that the thread
 // waits for one memory round trip; otherwise ub holds the velocities from wall_velocities.
-template <class S, class T, bool ONE_PASS> __device__ __forceinline__ void wall_node(const StepArgs<S, T> &a, int k)
+template <class S, class T, bool ONE_PASS, class UA = UGlobal<T>>
+__device__ __forceinline__ void wall_node(const StepArgs<S, T> &a, int k, UA ua = UA{})
 {
     using F = typename FormOf<T>::F;
     const BoundaryDev &b = a.b;
     T *U = (T *)b.U, *ub = (T *)b.ub;
-    const int nU = b.nU;
+    if constexpr (std::is_same_v<UA, UGlobal<T>>) ua = UGlobal<T>{U, b.nU, b.uc, b.uself};
     const T om = a.om;
     const bool interp_u = b.vel_interp || !b.rotated;
     const int i = b.ij[2 * k], j = b.ij[2 * k + 1];
@@ -893,15 +904,13 @@ template <class S, class T, bool ONE_PASS> __device__ __forceinline__ void wall_
     if constexpr (ONE_PASS) { // stencil u, v, Mxx', Myy' (every uc index is a stencil node)
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-            const int u = b.uc[12 * k + c];
-            cu[c] = U[nU + u];
-            cv[c] = U[2 * nU + u];
-            cm[c] = U[3 * nU + u];
-            cn[c] = U[4 * nU + u];
+            cu[c] = ua(k, 1, c);
+            cv[c] = ua(k, 2, c);
+            cm[c] = ua(k, 3, c);
+            cn[c] = ua(k, 4, c);
         }
-        const int u = b.uself[k];
-        su = U[nU + u];
-        sv = U[2 * nU + u];
+        su = ua(k, 1, 12);
+        sv = ua(k, 2, 12);
     }
     T fp[Q], ff[Q];
     gather_interior(a, i, j, fp);
@@ -980,7 +989,7 @@ template <class S, class T, bool ONE_PASS> __device__ __forceinline__ void wall_
             for (int r = 0; r < 3; r++) {
                 double pc[4];
                 for (int c = 0; c < 4; c++) {
-                    const double s0 = (double)U[b.uc[12 * k + 4 * r + c]];
+                    const double s0 = (double)ua(k, 0, 4 * r + c);
                     pc[c] = (F::dev ? 1.0 + s0 : s0) * (1.0 / 3.0);
                 }
                 pv[r] = bilin<double>(R[BT_XD2 + 2 * r], R[BT_YD2 + 2 * r], pc[0], pc[1], pc[2], pc[3]);
@@ -988,6 +997,30 @@ template <class S, class T, bool ONE_PASS> __device__ __forceinline__ void wall_
             b.pcy[k] = quadratic_fit(R[BT_R2], pv[0], R[BT_R3], pv[1], R[BT_R4], pv[2], R[BT_RS]);
         }
     }
+}
+
+// Wall group g (single wave, walls of one dependency level): the pre-collision moments of the
+// stencil nodes its boundary nodes read, into LDS (gather and moments as stencil_moments_body),
+// then WALL_GROUP boundary nodes as wall_nodes_kernel does them.  Reading the source state
+// only, a group needs no other workgroup of the step, so the groups run inside the side launch.
+template <class S, class T> __device__ void wall_group(const StepArgs<S, T> &a, int g)
+{
+    using F = typename FormOf<T>::F;
+    const BoundaryDev &b = a.b;
+    __shared__ T sU[5][WALL_GROUP_MAXU];
+    const int lane = threadIdx.x, off = b.wg_off[g], cnt = b.wg_off[g + 1] - off;
+    for (int t = lane; t < cnt; t += 64) {
+        const int u = b.wg_list[off + t];
+        T f[Q], m[6], v[5];
+        gather_interior(a, b.uij[2 * u], b.uij[2 * u + 1], f);
+        F::moments(f, m);
+        stencil_values(b, u, m, v);
+#pragma unroll
+        for (int p = 0; p < 5; p++) sU[p][t] = v[p];
+    }
+    __syncthreads();
+    const int q = g * WALL_GROUP + lane;
+    if (lane < WALL_GROUP && q < b.nbct) wall_node<S, T, true>(a, b.wg_node[q], ULocal<T>{sU, b.wl_slot});
 }
 
 // Weight of wall node q in the smoothed surface pressure at node t (2d_mrlbm.f90:722-741):
@@ -1481,6 +1514,35 @@ static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t t0
     else hipLaunchKernelGGL(kernel, g, b, 0, st, args...);
 }
 
+// Four-step strip heights built: FP32 12..32 (multiples of 4), FP64 8, 16, 20, 24 (at 12 rows
+// the compiler moves the FP64 march's carried rows to scratch memory).
+template <class S> bool bulkn_rows_ok(int H)
+{
+    if constexpr (sizeof(S) == 4) return H == 12 || H == 16 || H == 20 || H == 24 || H == 28 || H == 32;
+    else return H == 8 || H == 16 || H == 20 || H == 24;
+}
+// calls f with std::integral_constant<int, H> for a built strip height H (false otherwise)
+template <class S, class F> static bool with_bulkn_rows(int H, F &&f)
+{
+    if constexpr (sizeof(S) == 4) {
+        switch (H) {
+        case 12: f(std::integral_constant<int, 12>{}); return true;
+        case 16: f(std::integral_constant<int, 16>{}); return true;
+        case 20: f(std::integral_constant<int, 20>{}); return true;
+        case 24: f(std::integral_constant<int, 24>{}); return true;
+        case 28: f(std::integral_constant<int, 28>{}); return true;
+        case 32: f(std::integral_constant<int, 32>{}); return true;
+        }
+    } else {
+        switch (H) {
+        case 8: f(std::integral_constant<int, 8>{}); return true;
+        case 16: f(std::integral_constant<int, 16>{}); return true;
+        case 20: f(std::integral_constant<int, 20>{}); return true;
+        case 24: f(std::integral_constant<int, 24>{}); return true;
+        }
+    }
+    return false;
+}
 template <class S, class T>
 hipError_t launch_bulk(const StepArgs<S, T> &a, int nblocks, hipStream_t st, hipEvent_t t0, hipEvent_t t1)
 {
@@ -1608,35 +1670,6 @@ hipError_t launch_bulk2(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t
 #undef MRLBM_BULK2
     return hipGetLastError();
 }
-// Four-step strip heights built: FP32 12..32 (multiples of 4), FP64 8, 16, 20, 24 (at 12 rows
-// the compiler moves the FP64 march's carried rows to scratch memory).
-template <class S> bool bulkn_rows_ok(int H)
-{
-    if constexpr (sizeof(S) == 4) return H == 12 || H == 16 || H == 20 || H == 24 || H == 28 || H == 32;
-    else return H == 8 || H == 16 || H == 20 || H == 24;
-}
-// calls f with std::integral_constant<int, H> for a built strip height H (false otherwise)
-template <class S, class F> static bool with_bulkn_rows(int H, F &&f)
-{
-    if constexpr (sizeof(S) == 4) {
-        switch (H) {
-        case 12: f(std::integral_constant<int, 12>{}); return true;
-        case 16: f(std::integral_constant<int, 16>{}); return true;
-        case 20: f(std::integral_constant<int, 20>{}); return true;
-        case 24: f(std::integral_constant<int, 24>{}); return true;
-        case 28: f(std::integral_constant<int, 28>{}); return true;
-        case 32: f(std::integral_constant<int, 32>{}); return true;
-        }
-    } else {
-        switch (H) {
-        case 8: f(std::integral_constant<int, 8>{}); return true;
-        case 16: f(std::integral_constant<int, 16>{}); return true;
-        case 20: f(std::integral_constant<int, 20>{}); return true;
-        case 24: f(std::integral_constant<int, 24>{}); return true;
-        }
-    }
-    return false;
-}
 template <class S, class T>
 hipError_t launch_bulkn(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t st, hipEvent_t t0, hipEvent_t t1)
 {
@@ -1658,13 +1691,15 @@ template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipS
     const int nct = ring_column_tiles(a.nx, a.ny, a.pops != nullptr);
     const long long n = (nct ? 2 * a.nx : nring) + nband + (stencil && a.has_boundary ? a.b.nU : 0);
     // single-wave workgroups: they fit beside a running bulk launch (see wall_nodes_kernel)
-    hipLaunchKernelGGL((side_kernel<S, T>), dim3((unsigned)(nct + (n + 63) / 64)), dim3(64), 0, st, a, (int)nband);
+    const dim3 g((unsigned)(a.wall_groups + nct + (n + 63) / 64));
+    if (a.wall_groups) hipLaunchKernelGGL((side_kernel<S, T, true>), g, dim3(64), 0, st, a, (int)nband);
+    else hipLaunchKernelGGL((side_kernel<S, T, false>), g, dim3(64), 0, st, a, (int)nband);
     return hipGetLastError();
 }
 template <class S, class T> hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st)
 {
     if (a.b.nlevels <= 1) {
-        if (a.b.nbct > 0)
+        if (a.b.nbct > 0 && !a.wall_groups) // (else the side launch did the wall nodes)
             hipLaunchKernelGGL((wall_nodes_kernel<S, T>), dim3((a.b.nbct + 63) / 64), dim3(64), 0, st, a);
         if (a.rec_slot >= 0 || a.inlet_prev) {
             StepArgs<S, T> r = a;
