@@ -112,14 +112,13 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
     };
     for (int r = 0; r < R; r++) {
         mrlbm_handle *h = b->runs[r];
-        // side blocks: the ring's column tiles, then 64 items each (ring rows or the whole
-        // ring, bands, stencil nodes), as side_block counts them
+        // side blocks: the marched band rows, the ring's column tiles, then 64 items each (ring
+        // rows or the whole ring, the other bands, stencil nodes), as side_block counts them
         const int nct = ring_column_tiles(h->nx, h->ny, false);
         const int ring = 64 * nct + (nct ? 2 * h->nx : 2 * h->nx + 2 * (h->ny - 2)), nU = h->has_b ? h->bt.nU : 0;
-        auto band = [](const int rc[5][4], int n) {
-            int t = 0;
-            for (int q = 0; q < n; q++) t += rc[q][2] * rc[q][3];
-            return t;
+        auto band = [&](const int rc[5][4], int n) {
+            const int bm = band_march_rects(rc, n);
+            return (int)(64 * band_march_blocks(h->nx, bm) + band_items(rc, n, bm));
         };
         add_blocks(stiles[0], r, ring + nU);
         if (h->has_b) {
@@ -142,12 +141,14 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
                     x.dst = (S *)h->S2;
                     x.nrect = h->nrect1;
                     std::memcpy(x.rect, h->rect1, sizeof h->rect1);
+                    x.band_march = band_march_rects(x.rect, x.nrect);
                     s1[par][rec].push_back(x);
                     auto y = a; // sub-step 2: scratch -> state t+2
                     y.src = (const S *)h->S2;
                     y.dst = (S *)h->S[par ^ 1];
                     y.nrect = h->nrect2;
                     std::memcpy(y.rect, h->rect2, sizeof h->rect2);
+                    y.band_march = band_march_rects(y.rect, y.nrect);
                     s2[par][rec].push_back(y);
                     if (rec == 0) {
                         auto z = one; // two-step bulk: state t -> state t+2
@@ -166,6 +167,7 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
                         x.dst = (S *)dst[k];
                         x.nrect = h->nrectq[k];
                         std::memcpy(x.rect, h->rectq[k], sizeof h->rectq[k]);
+                        x.band_march = band_march_rects(x.rect, x.nrect);
                         q[k][par][rec].push_back(x);
                     }
                     if (rec == 0) {

@@ -863,6 +863,7 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
                 a.skip_ns = 4;
                 a.nrect = h->nrectq[k];
                 std::memcpy(a.rect, h->rectq[k], sizeof h->rectq[k]);
+                a.band_march = band_march_rects(a.rect, a.nrect);
                 HIPCHK(side_work(a));
                 after_iter(iter + k);
             }
@@ -889,6 +890,7 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             HIPCHK(launch_bulk_pair(bulk));
             a1.nrect = h->nrect1;
             std::memcpy(a1.rect, h->rect1, sizeof h->rect1);
+            a1.band_march = band_march_rects(a1.rect, a1.nrect);
             const int slot2 = is_window(iter + 1) ? slot1 + (is_window(iter) ? 1 : 0) : -1;
             if (opts.nprobe > 0 && (a1.rec_slot >= 0 || slot2 >= 0)) {
                 // the probes the bulk carries; the next pair's side work (which overwrites
@@ -902,6 +904,7 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
             auto a2 = step_args(iter + 1, h->S2, C, slot1 + (is_window(iter) ? 1 : 0));
             a2.nrect = h->nrect2;
             std::memcpy(a2.rect, h->rect2, sizeof h->rect2);
+            a2.band_march = band_march_rects(a2.rect, a2.nrect);
             HIPCHK(side_work(a2));
             if (a1.sample && bulk.sample2) { // after the bulk and sub-step 1 (see the kernel)
                 HIPCHK(hipStreamWaitEvent(h->st2, h->bulk_done, 0));

@@ -72,11 +72,37 @@ template <class S, class T> struct StepArgs {
     int wall_groups;           // side kernel: the first wall_groups blocks are wall groups (stencil
                                // moments in LDS, then the wall nodes); 0: the stencil nodes are
                                // side items and the wall nodes a launch of their own
+    int band_march;            // side kernel: band rectangles 0 .. band_march-1 (the rows next to the
+                               // channel walls) go by row marches, not node by node
     int nrect;                 // side kernel: band rectangles x0, y0, w, h
     int rect[6][4];
     const T *chi;              // Brinkman solid fraction (per-node bulk kernel only), or null
     BoundaryDev b;
 };
+
+// Band rectangles 0 and 1 of a multi-step sub-step are the rows next to the channel walls
+// (rows 1..L and ny-1-L..ny-2, columns 1..nx-2).  When the cylinder's rectangle (4) meets
+// neither, they hold only plain interior nodes and the side launch marches them, 62
+// columns per single-wave workgroup: how many rectangles go that way (2 or 0), and the
+// workgroups they take.
+__host__ __device__ inline int band_march_rects(const int rc[][4], int n)
+{
+    if (n < 2) return 0;
+    for (int q = 4; q < n; q++)
+        for (int r = 0; r < 2; r++)
+            if (rc[q][0] < rc[r][0] + rc[r][2] && rc[r][0] < rc[q][0] + rc[q][2] && rc[q][1] < rc[r][1] + rc[r][3] &&
+                rc[r][1] < rc[q][1] + rc[q][3])
+                return 0;
+    return 2;
+}
+__host__ __device__ inline int band_march_blocks(int nx, int nrects) { return nrects * ((nx - 2 + 61) / 62); }
+// Band nodes the side launch takes node by node (rectangles from the first not marched).
+__host__ __device__ inline long long band_items(const int rc[][4], int n, int marched)
+{
+    long long t = 0;
+    for (int q = marched; q < n; q++) t += (long long)rc[q][2] * rc[q][3];
+    return t;
+}
 
 // Boundary nodes per wall group, and the most stencil nodes a group may read (LDS slots).
 constexpr int WALL_GROUP = 32, WALL_GROUP_MAXU = 256;

@@ -104,7 +104,7 @@ __device__ __forceinline__ void march_row(const StepArgs<S, T> &a, int c, bool v
                 m[2] = (T)0.0;
             }
             const long long n = (long long)j * a.pitch + c;
-            if (REC) {
+            if (REC && records_node(a, c, j)) {
                 if (a.rec_slot >= 0)
                     for (int p = 0; p < a.nprobe; p++)
                         if (n == a.probe_node[p]) {
@@ -725,7 +725,7 @@ template <class S, class T> __device__ void side_body(const StepArgs<S, T> &a, l
     }
     t -= nring;
     if (t < nband) {
-        for (int q = 0; q < a.nrect; q++) {
+        for (int q = a.band_march; q < a.nrect; q++) {
             const long long n = (long long)a.rect[q][2] * a.rect[q][3];
             if (t < n) { // every rectangle lies in [1, nx-2] x [1, ny-2]: all 8 neighbours inside
                 const int i = a.rect[q][0] + (int)(t % a.rect[q][2]), j = a.rect[q][1] + (int)(t / a.rect[q][2]);
@@ -745,9 +745,37 @@ template <class S, class T> __device__ void side_body(const StepArgs<S, T> &a, l
 
 template <class S, class T> __device__ __forceinline__ int band_nodes(const StepArgs<S, T> &a)
 {
-    int n = 0;
-    for (int q = 0; q < a.nrect; q++) n += a.rect[q][2] * a.rect[q][3];
-    return n;
+    return (int)band_items(a.rect, a.nrect, a.band_march);
+}
+
+// Band rectangle q (0 or 1: the rows y0 .. y0+L-1 next to a channel wall, columns 1..nx-2,
+// only plain interior nodes) by a row march, one single wave per 62 columns: lanes 1..62
+// write columns 62g+1 .. 62g+62, lanes 0 and 63 carry the halo columns, and each row is
+// the bulk march's single-step row update (march_row) from rows loaded one ahead.
+template <class S, class T> __device__ void band_march(const StepArgs<S, T> &a, int blk)
+{
+    using F = typename FormOf<T>::F;
+    const int per = (a.nx - 2 + 61) / 62, q = blk / per, g = blk - q * per;
+    const int lane = threadIdx.x, c = 62 * g + lane, cl = min(c, a.nx - 1);
+    const bool valid = lane >= 1 && lane <= 62 && c <= a.nx - 2;
+    const int y0 = a.rect[q][1], y1 = y0 + a.rect[q][3]; // rows y0 .. y1-1; y0-1 and y1 are in the grid
+    S mA[6], mB[6];
+    T F9[Q], up_prev[3], up_cur[3], mid_cur[3];
+    load_moments(a, cl, y0 - 1, mA);
+    F::recon(mA, F9);
+    up_prev[0] = F9[2]; up_prev[1] = F9[5]; up_prev[2] = F9[6];
+    load_moments(a, cl, y0, mA);
+    F::recon(mA, F9);
+    up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
+    mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
+    load_moments(a, cl, y0 + 1, mA);
+    for (int j = y0; j < y1; j += 2) {
+        if (j + 2 <= y1) load_moments(a, cl, j + 2, mB);
+        march_row<S, T, true>(a, c, valid, j, mA, up_prev, up_cur, mid_cur, 0);
+        if (j + 1 >= y1) break;
+        if (j + 3 <= y1) load_moments(a, cl, j + 3, mA);
+        march_row<S, T, true>(a, c, valid, j + 1, mB, up_prev, up_cur, mid_cur, 0);
+    }
 }
 
 // The inlet and outlet columns of the outer ring (rows 1 .. ny-2) in tiles of 64 rows, one
@@ -819,6 +847,14 @@ template <class S, class T, bool WG = false> __device__ void side_block(const St
             return;
         }
         blk -= a.wall_groups;
+    }
+    if (a.band_march) {
+        const int nbm = band_march_blocks(a.nx, a.band_march);
+        if (blk < nbm) {
+            band_march(a, blk);
+            return;
+        }
+        blk -= nbm;
     }
     const int nct = ring_column_tiles(a.nx, a.ny, a.pops != nullptr);
     if (blk < nct) {
@@ -1686,12 +1722,11 @@ hipError_t launch_bulkn(const StepArgs<S, T> &a, int nblocks, int H, hipStream_t
 template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st, bool stencil)
 {
     const int nring = 2 * a.nx + 2 * (a.ny - 2);
-    long long nband = 0;
-    for (int q = 0; q < a.nrect; q++) nband += (long long)a.rect[q][2] * a.rect[q][3];
+    const long long nband = band_items(a.rect, a.nrect, a.band_march);
     const int nct = ring_column_tiles(a.nx, a.ny, a.pops != nullptr);
     const long long n = (nct ? 2 * a.nx : nring) + nband + (stencil && a.has_boundary ? a.b.nU : 0);
     // single-wave workgroups: they fit beside a running bulk launch (see wall_nodes_kernel)
-    const dim3 g((unsigned)(a.wall_groups + nct + (n + 63) / 64));
+    const dim3 g((unsigned)(a.wall_groups + band_march_blocks(a.nx, a.band_march) + nct + (n + 63) / 64));
     if (a.wall_groups) hipLaunchKernelGGL((side_kernel<S, T, true>), g, dim3(64), 0, st, a, (int)nband);
     else hipLaunchKernelGGL((side_kernel<S, T, false>), g, dim3(64), 0, st, a, (int)nband);
     return hipGetLastError();
