@@ -118,7 +118,7 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
         const int ring = 64 * nct + (nct ? 2 * h->nx : 2 * h->nx + 2 * (h->ny - 2)), nU = h->has_b ? h->bt.nU : 0;
         auto band = [&](const int rc[5][4], int n) {
             const int bm = band_march_rects(rc, n);
-            return (int)(64 * band_march_blocks(h->nx, bm) + band_items(rc, n, bm));
+            return (int)(64 * band_march_blocks(rc, n, bm) + band_items(rc, n, bm));
         };
         add_blocks(stiles[0], r, ring + nU);
         if (h->has_b) {

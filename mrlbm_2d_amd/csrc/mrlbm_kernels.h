@@ -72,35 +72,41 @@ template <class S, class T> struct StepArgs {
     int wall_groups;           // side kernel: the first wall_groups blocks are wall groups (stencil
                                // moments in LDS, then the wall nodes); 0: the stencil nodes are
                                // side items and the wall nodes a launch of their own
-    int band_march;            // side kernel: band rectangles 0 .. band_march-1 (the rows next to the
-                               // channel walls) go by row marches, not node by node
+    int band_march;            // side kernel: bit q set = band rectangle q goes by row marches, not
+                               // node by node
     int nrect;                 // side kernel: band rectangles x0, y0, w, h
     int rect[6][4];
     const T *chi;              // Brinkman solid fraction (per-node bulk kernel only), or null
     BoundaryDev b;
 };
 
-// Band rectangles 0 and 1 of a multi-step sub-step are the rows next to the channel walls
-// (rows 1..L and ny-1-L..ny-2, columns 1..nx-2).  When the cylinder's rectangle (4) meets
-// neither, they hold only plain interior nodes and the side launch marches them, 62
-// columns per single-wave workgroup: how many rectangles go that way (2 or 0), and the
-// workgroups they take.
-__host__ __device__ inline int band_march_rects(const int rc[][4], int n)
+// The band rectangles of a multi-step sub-step: 0 and 1 the rows next to the channel walls
+// (rows 1..L and ny-1-L..ny-2, columns 1..nx-2), 2 and 3 the columns next to the inlet and
+// outlet (L wide), 4 the cylinder's rectangle.  The side launch marches 0, 1 and 4 row by
+// row (62 columns and up to BAND_STRIP rows per single-wave workgroup) and takes the narrow
+// 2 and 3 node by node.  The marched rectangles (a bit mask), the workgroups rectangle q's
+// march takes, all marched workgroups, and the nodes taken node by node.
+constexpr int BAND_STRIP = 16;
+__host__ __device__ inline int band_march_rects(const int (*)[4], int n)
 {
-    if (n < 2) return 0;
-    for (int q = 4; q < n; q++)
-        for (int r = 0; r < 2; r++)
-            if (rc[q][0] < rc[r][0] + rc[r][2] && rc[r][0] < rc[q][0] + rc[q][2] && rc[q][1] < rc[r][1] + rc[r][3] &&
-                rc[r][1] < rc[q][1] + rc[q][3])
-                return 0;
-    return 2;
+    return (n >= 2 ? 3 : 0) | (n >= 5 ? 16 : 0);
 }
-__host__ __device__ inline int band_march_blocks(int nx, int nrects) { return nrects * ((nx - 2 + 61) / 62); }
-// Band nodes the side launch takes node by node (rectangles from the first not marched).
+__host__ __device__ inline int band_march_rect_blocks(const int rc[4])
+{
+    return ((rc[2] + 61) / 62) * ((rc[3] + BAND_STRIP - 1) / BAND_STRIP);
+}
+__host__ __device__ inline int band_march_blocks(const int rc[][4], int n, int marched)
+{
+    int b = 0;
+    for (int q = 0; q < n; q++)
+        if ((marched >> q) & 1) b += band_march_rect_blocks(rc[q]);
+    return b;
+}
 __host__ __device__ inline long long band_items(const int rc[][4], int n, int marched)
 {
     long long t = 0;
-    for (int q = marched; q < n; q++) t += (long long)rc[q][2] * rc[q][3];
+    for (int q = 0; q < n; q++)
+        if (!((marched >> q) & 1)) t += (long long)rc[q][2] * rc[q][3];
     return t;
 }
 

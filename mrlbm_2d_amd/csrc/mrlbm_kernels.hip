@@ -725,7 +725,8 @@ template <class S, class T> __device__ void side_body(const StepArgs<S, T> &a, l
     }
     t -= nring;
     if (t < nband) {
-        for (int q = a.band_march; q < a.nrect; q++) {
+        for (int q = 0; q < a.nrect; q++) {
+            if ((a.band_march >> q) & 1) continue; // marched (band_march)
             const long long n = (long long)a.rect[q][2] * a.rect[q][3];
             if (t < n) { // every rectangle lies in [1, nx-2] x [1, ny-2]: all 8 neighbours inside
                 const int i = a.rect[q][0] + (int)(t % a.rect[q][2]), j = a.rect[q][1] + (int)(t / a.rect[q][2]);
@@ -748,17 +749,32 @@ template <class S, class T> __device__ __forceinline__ int band_nodes(const Step
     return (int)band_items(a.rect, a.nrect, a.band_march);
 }
 
-// Band rectangle q (0 or 1: the rows y0 .. y0+L-1 next to a channel wall, columns 1..nx-2,
-// only plain interior nodes) by a row march, one single wave per 62 columns: lanes 1..62
-// write columns 62g+1 .. 62g+62, lanes 0 and 63 carry the halo columns, and each row is
-// the bulk march's single-step row update (march_row) from rows loaded one ahead.
+// Marched band rectangle: workgroup blk of the marched rectangles (in rectangle order, each
+// cut into 62-column groups x BAND_STRIP-row strips), one single wave: lanes 1..62 write
+// columns x0+62g .. x0+62g+61 of rows y0 .. y1-1, lanes 0 and 63 carry the halo columns, and
+// each row is the bulk march's single-step row update (march_row) from rows loaded one
+// ahead.  Node classes come from the cylinder's box (boundary nodes are the wall's, solid
+// nodes are zeroed on plot steps); nodes of an earlier rectangle are left to it.
 template <class S, class T> __device__ void band_march(const StepArgs<S, T> &a, int blk)
 {
     using F = typename FormOf<T>::F;
-    const int per = (a.nx - 2 + 61) / 62, q = blk / per, g = blk - q * per;
-    const int lane = threadIdx.x, c = 62 * g + lane, cl = min(c, a.nx - 1);
-    const bool valid = lane >= 1 && lane <= 62 && c <= a.nx - 2;
-    const int y0 = a.rect[q][1], y1 = y0 + a.rect[q][3]; // rows y0 .. y1-1; y0-1 and y1 are in the grid
+    int q = 0;
+    for (; q < a.nrect; q++) { // the rectangle of this workgroup
+        if (!((a.band_march >> q) & 1)) continue;
+        const int nb = band_march_rect_blocks(a.rect[q]);
+        if (blk < nb) break;
+        blk -= nb;
+    }
+    const int x0 = a.rect[q][0], w = a.rect[q][2], per = (w + 61) / 62;
+    const int g = blk % per, st = blk / per;
+    const int y0 = a.rect[q][1] + st * BAND_STRIP, y1 = min(y0 + BAND_STRIP, a.rect[q][1] + a.rect[q][3]);
+    const int lane = threadIdx.x, c = x0 - 1 + 62 * g + lane, cl = min(c, a.nx - 1);
+    const bool lanev = lane >= 1 && lane <= 62 && c < x0 + w; // rows y0-1 .. y1 lie in the grid
+    auto cls_of = [&](int j) {
+        const int bi = c - a.bx0, bj = j - a.by0;
+        return ((unsigned)bi < (unsigned)a.bw && (unsigned)bj < (unsigned)a.bh) ? (int)a.boxcls[bj * a.bw + bi] : 0;
+    };
+    auto valid = [&](int j) { return lanev && !in_rects(a, q, c, j); };
     S mA[6], mB[6];
     T F9[Q], up_prev[3], up_cur[3], mid_cur[3];
     load_moments(a, cl, y0 - 1, mA);
@@ -771,10 +787,10 @@ template <class S, class T> __device__ void band_march(const StepArgs<S, T> &a, 
     load_moments(a, cl, y0 + 1, mA);
     for (int j = y0; j < y1; j += 2) {
         if (j + 2 <= y1) load_moments(a, cl, j + 2, mB);
-        march_row<S, T, true>(a, c, valid, j, mA, up_prev, up_cur, mid_cur, 0);
+        march_row<S, T, true>(a, c, valid(j), j, mA, up_prev, up_cur, mid_cur, cls_of(j));
         if (j + 1 >= y1) break;
         if (j + 3 <= y1) load_moments(a, cl, j + 3, mA);
-        march_row<S, T, true>(a, c, valid, j + 1, mB, up_prev, up_cur, mid_cur, 0);
+        march_row<S, T, true>(a, c, valid(j + 1), j + 1, mB, up_prev, up_cur, mid_cur, cls_of(j + 1));
     }
 }
 
@@ -849,7 +865,7 @@ template <class S, class T, bool WG = false> __device__ void side_block(const St
         blk -= a.wall_groups;
     }
     if (a.band_march) {
-        const int nbm = band_march_blocks(a.nx, a.band_march);
+        const int nbm = band_march_blocks(a.rect, a.nrect, a.band_march);
         if (blk < nbm) {
             band_march(a, blk);
             return;
@@ -1726,7 +1742,7 @@ template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipS
     const int nct = ring_column_tiles(a.nx, a.ny, a.pops != nullptr);
     const long long n = (nct ? 2 * a.nx : nring) + nband + (stencil && a.has_boundary ? a.b.nU : 0);
     // single-wave workgroups: they fit beside a running bulk launch (see wall_nodes_kernel)
-    const dim3 g((unsigned)(a.wall_groups + band_march_blocks(a.nx, a.band_march) + nct + (n + 63) / 64));
+    const dim3 g((unsigned)(a.wall_groups + band_march_blocks(a.rect, a.nrect, a.band_march) + nct + (n + 63) / 64));
     if (a.wall_groups) hipLaunchKernelGGL((side_kernel<S, T, true>), g, dim3(64), 0, st, a, (int)nband);
     else hipLaunchKernelGGL((side_kernel<S, T, false>), g, dim3(64), 0, st, a, (int)nband);
     return hipGetLastError();
