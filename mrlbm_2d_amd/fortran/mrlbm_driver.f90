@@ -26,6 +26,7 @@ program mrlbm_gpu
     logical :: strong_mass_bc, eq_mass_bc, weak_mass_bc
     integer :: precision = 0, inlet_profile = 0, strip_rows = 0, kernel = 0, time_blocking = -1
     integer :: steps_per_pass = 0
+    integer :: quad_rows = 0
     logical :: record_surface_p = .true.
 
     integer :: i_probe(4), j_probe(4)
@@ -87,6 +88,7 @@ program mrlbm_gpu
     par%kernel = kernel
     par%time_blocking = time_blocking
     par%steps_per_pass = steps_per_pass
+    par%quad_rows = quad_rows
     rc = mrlbm_create(par, h)
     if (rc /= 0) then
         write(*, *) 'mrlbm_create failed: ', trim(mrlbm_error_string(c_null_ptr))
@@ -428,7 +430,7 @@ contains
         namelist/LogicalControls/post_process, x_periodic, y_periodic, channel_with_cylinder, channel_with_square, &
             & incomp_bc, incomp_bulk, vel_interp, mom_interp, rotated_coordinate, strong_mass_bc, eq_mass_bc, weak_mass_bc
         namelist/Accelerator/precision, inlet_profile, strip_rows, kernel, record_surface_p, time_blocking, &
-            & steps_per_pass
+            & steps_per_pass, quad_rows
 
         open(newunit=u, file="input.dat", status="old", iostat=ios)
         if (ios /= 0) then
