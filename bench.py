@@ -15,7 +15,8 @@ traffic = one launch's HBM bytes from rocprofv3 PMC counters (FETCH_SIZE, WRITE_
 committed profile named in `traffic_source`, not measured in this run; below the algorithmic
 bytes because the intermediate steps never leave the chip, so `frac` (algorithmic) passes 1
 and `traffic_frac` (those bytes over the same launch time, against 8 TB/s) is the share of HBM
-bandwidth the launch actually uses.
+bandwidth the launch actually uses; `valu_issue` (same profile) is the share of the SIMDs'
+cycles the four-step launch spends issuing VALU instructions, the bound it runs into.
 `statistics_window`: ms per step inside the statistics window (forces, surface pressure,
 probes and u statistics recorded every step) against plain steps, both forms.
 `fp32_vs_fp64` is the accuracy condition of the FP32 figure: both forms from the same seeded
@@ -50,6 +51,9 @@ TRAFFIC = {
     4: ({"f32": 1_070_583_778, "f64": 2_017_994_926}, "profiles/bulkn_kernel_8192x2048_hbm_bytes.md", (20, 20)),
     2: ({"f32": 949_550_578, "f64": 2_289_678_504}, "profiles/bulk2_kernel_8192x2048_hbm_bytes.md", (10, 12)),
 }
+# Share of the SIMDs' cycles issuing VALU instructions during a four-step launch (the bound of
+# that kernel), from the same profile's SQ_INSTS_VALU x 4 cycles over GRBM_GUI_ACTIVE
+VALU_ISSUE = {4: ({"f32": 0.918, "f64": 0.848}, "profiles/bulkn_kernel_8192x2048_hbm_bytes.md", (20, 20))}
 NX, NY = 8192, 2048
 # HIP event pairs around every 4th bulk launch on long runs: the per-launch mean without the
 # events' own cost (about 2% of the step when recorded around every launch); runs of up to
@@ -143,11 +147,12 @@ def run_form(precision, steps, warmup, gate_steps=0, gate_window=0, **kw):
                 first_timed_step=it, pair_rows=info.pair_rows, quad_rows=info.quad_rows), out
 
 
-def roofline(r, kernel, traffic, source):
+def roofline(r, kernel, traffic, source, valu=None):
     """Roofline of the dominant kernel: algorithmic bytes of the timed launches (12 moment
     words per node and step) over their summed duration, against 8 TB/s; traffic = HBM bytes
     of one launch from the committed PMC profile (None for a launch shape not profiled), and
-    traffic_frac = those bytes over the mean launch time, against 8 TB/s."""
+    traffic_frac = those bytes over the mean launch time, against 8 TB/s; valu_issue = the
+    profiled share of SIMD cycles issuing VALU instructions (the four-step kernel's bound)."""
     tgbs = traffic / (r["launch_ms"] * 1e-3) / 1e9 if traffic else None
     return {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(r["frac"], 4), "traffic": traffic,
@@ -155,6 +160,9 @@ def roofline(r, kernel, traffic, source):
                                "this run)") if traffic else None,
             "traffic_gbs": round(tgbs, 1) if tgbs else None,
             "traffic_frac": round(tgbs / HBM_PEAK_GBS, 4) if tgbs else None,
+            "valu_issue": valu,
+            "valu_source": (source + " (rocprofv3 --pmc SQ_INSTS_VALU, GRBM_GUI_ACTIVE; not measured in this "
+                            "run)") if valu else None,
             "kernel": kernel, "steps_per_launch": r["steps_per_launch"],
             "algorithmic_bytes_per_launch": int(round(r["steps_per_launch"] * r["bytes_per_step"])),
             "launch_ms": round(r["launch_ms"], 5), "launch_median_ms": round(r["launch_median_ms"], 5),
@@ -423,9 +431,10 @@ def main():
         k64 = f"bulk2_kernel<double,double,{r64['pair_rows']},false>"
     # measured traffic applies to the profiled launch shapes only
     traffic, source, shape = TRAFFIC.get(spp, ({"f32": None, "f64": None}, None, None))
+    valu = VALU_ISSUE.get(spp, ({"f32": None, "f64": None}, None, None))[0]
     rows = (r32["quad_rows"], r64["quad_rows"]) if spp == 4 else (r32["pair_rows"], r64["pair_rows"])
     if args.strip_rows != 0 or tb not in (-1,) or rows != shape:
-        traffic = {"f32": None, "f64": None}
+        traffic = valu = {"f32": None, "f64": None}
     out = {
         # the headline metric: MLUPS of the FP32 deviation form (value), its % of HBM peak
         # (roofline), the FP64 form beside it, the |dCd|/Cd condition (fp32_vs_fp64) and the
@@ -444,9 +453,9 @@ def main():
         "data": "synthetic (seeded +-1e-4 velocity noise)",
         "config": {"workload": "8192x2048 channel, cylinder D=128 at (1024,1025), "
                                "parabolic inlet u_max=0.05, Re=200", "nx": NX, "ny": NY},
-        "roofline": roofline(r32, k32, traffic["f32"], source),
+        "roofline": roofline(r32, k32, traffic["f32"], source, valu["f32"]),
         "fp64": {"value": round(r64["mlups"], 1), "ms_per_step": round(r64["ms_per_step"], 5),
-                 "roofline": roofline(r64, k64, traffic["f64"], source)},
+                 "roofline": roofline(r64, k64, traffic["f64"], source, valu["f64"])},
         "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3),
         "fp32_vs_fp64": gate,
         # the timed steps continue each form's accuracy run from this step number

@@ -37,6 +37,11 @@ def test_roofline_fields_and_arithmetic():
         assert tr["f32"] > 0 and tr["f64"] > 0, spp
     none = bench.roofline(r, "k", None, None)
     assert none["traffic_source"] is None and none["traffic_frac"] is None
+    assert none["valu_issue"] is None and none["valu_source"] is None
+    shares, src, _ = bench.VALU_ISSUE[4]
+    with_valu = bench.roofline(r, "k", traffic["f32"], src, shares["f32"])
+    assert 0.5 < with_valu["valu_issue"] < 1.0 and with_valu["valu_source"].startswith(src)
+    assert os.path.exists(os.path.join(REPO, src))
     json.dumps(out)
 
 
