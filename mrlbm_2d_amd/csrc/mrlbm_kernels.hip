@@ -57,10 +57,11 @@ __device__ __forceinline__ int row_cls(unsigned long long bm, unsigned long long
 
 // Whole-wave shifts by one lane, as DPP moves in the VALU (wave_shr:1 / wave_shl:1) rather
 // than LDS permutes: lane i receives lane i-1 (shup) or lane i+1 (shdn); lane 0 (shup) and
-// lane 63 (shdn) receive 0, and the marches never use those lanes' shifted values.
+// lane 63 (shdn) receive 0 (bound_ctrl: a lane without a source reads 0, so no register
+// has to be cleared first), and the marches never use those lanes' shifted values.
 template <int CTRL> __device__ __forceinline__ int dpp_shift(int v)
 {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true);
 }
 template <class T, int CTRL> __device__ __forceinline__ T dpp_shift_t(T v)
 {
@@ -70,6 +71,28 @@ template <class T, int CTRL> __device__ __forceinline__ T dpp_shift_t(T v)
         const long long x = __builtin_bit_cast(long long, v);
         const int lo = dpp_shift<CTRL>((int)(x & 0xffffffffll)), hi = dpp_shift<CTRL>((int)(x >> 32));
         return __builtin_bit_cast(T, ((long long)hi << 32) | (long long)(unsigned)lo);
+    }
+}
+// Raw buffer access for the marches: one resource per plane and wave (scalar registers), a
+// 32-bit byte offset per lane and one per row (scalar), so a row load or store needs no
+// vector address arithmetic.  Word 3 is the gfx9 raw-buffer format (32-bit data); the
+// record count is the largest, so no access is dropped by the range check.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, -1, 0x00020000);
+}
+template <class S> __device__ __forceinline__ S buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(S) == 4) return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    else return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <class S> __device__ __forceinline__ void buf_store(S v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(S) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+    } else {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
     }
 }
 template <class T> __device__ __forceinline__ T shup(T v) { return dpp_shift_t<T, 0x138>(v); }
@@ -135,7 +158,7 @@ __device__ __forceinline__ void march_row(const StepArgs<S, T> &a, int c, bool v
 // column per side) marching up a strip of rows.  The loop is unrolled by two with two
 // load buffers, so the loads of row j+2 are in flight while row j is computed and no
 // register copy ever waits on them.
-template <class S, class T, bool REC> __device__ void bulk_march(const StepArgs<S, T> &a, int bb)
+template <class S, class T, bool REC> __device__ __forceinline__ void bulk_march(const StepArgs<S, T> &a, int bb)
 {
     using F = typename FormOf<T>::F;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -171,7 +194,7 @@ template <class S, class T, bool REC> __device__ void bulk_march(const StepArgs<
 // Same march with the strip height a compile-time constant and the row loop fully
 // unrolled: every load of the strip is independent straight-line code, so the compiler
 // keeps many rows in flight without loop-carried copies of loaded registers.
-template <class S, class T, bool REC, int H> __device__ void bulk_march_u(const StepArgs<S, T> &a, int bb)
+template <class S, class T, bool REC, int H> __device__ __forceinline__ void bulk_march_u(const StepArgs<S, T> &a, int bb)
 {
     using F = typename FormOf<T>::F;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -268,7 +291,7 @@ __device__ __forceinline__ void march_row_a(const StepArgs<S, T> &a, int c, bool
     hm_cur[0] = hm_next;
 }
 
-template <class S, class T, bool REC, int H> __device__ void bulk_march_a(const StepArgs<S, T> &a, int bb)
+template <class S, class T, bool REC, int H> __device__ __forceinline__ void bulk_march_a(const StepArgs<S, T> &a, int bb)
 {
     using F = typename FormOf<T>::F;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -399,7 +422,7 @@ template <class S, class T> __device__ __forceinline__ bool side_node1(const Ste
 // t+1 sampled as well, only those that sub-step 1 does not write).  Where the pass takes
 // both samples of a node, the t+1 velocity is carried one row in registers and both are
 // added in one read-modify-write (sample_u_pair).
-template <class S, class T, int H, bool SAMPLE> __device__ void bulk_march2(const StepArgs<S, T> &a, int bb)
+template <class S, class T, int H, bool SAMPLE> __device__ __forceinline__ void bulk_march2(const StepArgs<S, T> &a, int bb)
 {
     using F = typename FormOf<T>::F;
     constexpr int NQ = H + 4;             // state-t rows j0-2 .. j0+H+1
@@ -575,7 +598,7 @@ __global__ void __launch_bounds__(256) bulk2_kernel(StepArgs<S, T> a)
 // would; the sub-steps of the pass leave those nodes to it (records_node).  A node's level-s
 // velocity is made NS-s march steps before its level-NS one, so the wave carries it that
 // long, then adds the sampled levels in order in one read-modify-write of the accumulators.
-template <class S, class T, int NS, int H, bool REC> __device__ void bulk_march_n(const StepArgs<S, T> &a, int bb)
+template <class S, class T, int NS, int H, bool REC> __device__ __forceinline__ void bulk_march_n(const StepArgs<S, T> &a, int bb)
 {
     using F = typename FormOf<T>::F;
     // rows of state t loaded ahead: FP32 4, FP64 1 (FP64 at 2 spills at two waves per SIMD;
@@ -590,10 +613,21 @@ template <class S, class T, int NS, int H, bool REC> __device__ void bulk_march_
     const S *__restrict__ src = a.src; // the pass writes another state than it reads
     S *__restrict__ dst = a.dst;
     const long long plane = a.plane;
-    auto load_row = [&](int p, S m[6]) {
-        const long long n = (long long)min(j0 - NS + p, a.ny - 1) * a.pitch + cl;
+    // Buffer resources per plane based at the strip's first state-t row (which is <= ny-1):
+    // row offsets within the strip and lane offsets within the row fit 32 bits at any grid size.
+    const long long r0 = (long long)(j0 - NS) * a.pitch;
+    __amdgpu_buffer_rsrc_t rs[6], rd[6];
 #pragma unroll
-        for (int k = 0; k < 6; k++) m[k] = src[k * plane + n];
+    for (int k = 0; k < 6; k++) {
+        rs[k] = plane_rsrc(src + k * plane + r0);
+        rd[k] = plane_rsrc(dst + k * plane + r0);
+    }
+    const unsigned lb = (unsigned)cl * (unsigned)sizeof(S);
+    const unsigned rowb = (unsigned)a.pitch * (unsigned)sizeof(S);
+    auto load_row = [&](int p, S m[6]) {
+        const unsigned ro = (unsigned)(min(j0 - NS + p, a.ny - 1) - (j0 - NS)) * rowb;
+#pragma unroll
+        for (int k = 0; k < 6; k++) m[k] = buf_load<S>(rs[k], lb, ro);
     };
     S q[LA + 1][6];
 #pragma unroll
@@ -658,8 +692,9 @@ template <class S, class T, int NS, int H, bool REC> __device__ void bulk_march_
                 const int j = j0 + p - 2 * NS;
                 if (outc && n_step_node<S, T, NS>(a, c, j)) {
                     const long long n = (long long)j * a.pitch + c;
+                    const unsigned ro = (unsigned)(j - (j0 - NS)) * rowb;
 #pragma unroll
-                    for (int k = 0; k < 6; k++) dst[k * plane + n] = (S)m[k];
+                    for (int k = 0; k < 6; k++) buf_store<S>((S)m[k], rd[k], lb, ro);
                     if constexpr (REC) {
                         if (a.qsample) {
                             double s0 = a.acc[n], s1 = a.acc[plane + n], s2 = a.acc[2 * plane + n],

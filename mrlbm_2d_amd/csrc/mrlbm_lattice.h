@@ -91,36 +91,46 @@ __device__ __forceinline__ void moments_f64(const double f[Q], double &r, double
 __device__ __forceinline__ void recon9_dev(float d, float ux, float uy, float mxx, float myy, float mxy,
                                            float e[Q])
 {
-    // e_k = w_k (d + r T_k), r = 1 + d, T_k = 3 c_k.u + 4.5 H_k:m: the 4.5 H_aa products are
-    // single constants and the diagonal velocity terms are sums of the axis ones.
+    // e_k = w_k d + r w_k T_k, r = 1 + d, T_k = 3 c_k.u + 4.5 H_k:m.  Opposite directions
+    // (1,3), (2,4), (5,7), (6,8) share the even part w_k (d + r 4.5 H_k:m) and differ in the
+    // sign of the odd part 3 w_k r c_k.u, so each pair costs one even and one odd term; the
+    // weights are folded into the constants (35 operations for the nine values).
+    constexpr float w0 = Lat<float>::w0, w1 = Lat<float>::w1, w5 = Lat<float>::w5;
     constexpr float KA = 4.5f * Lat<float>::HA, KB = 4.5f * Lat<float>::HB;
-    const float XA = mxx * KA, XB = mxx * KB, YA = myy * KA, YB = myy * KB, Z = 9.0f * mxy;
-    const float c1 = 3.0f * ux, c2 = 3.0f * uy, c5 = c1 + c2, c6 = c2 - c1;
-    const float r = 1.0f + d;
-    const float P = XB + YA, Qv = XA + YB, B = XB + YB, Bp = B + Z, Bm = B - Z;
-    e[0] = Lat<float>::w0 * __builtin_fmaf(r, XA + YA, d);
-    e[1] = Lat<float>::w1 * __builtin_fmaf(r, P + c1, d);
-    e[2] = Lat<float>::w1 * __builtin_fmaf(r, Qv + c2, d);
-    e[3] = Lat<float>::w1 * __builtin_fmaf(r, P - c1, d);
-    e[4] = Lat<float>::w1 * __builtin_fmaf(r, Qv - c2, d);
-    e[5] = Lat<float>::w5 * __builtin_fmaf(r, Bp + c5, d);
-    e[6] = Lat<float>::w5 * __builtin_fmaf(r, Bm + c6, d);
-    e[7] = Lat<float>::w5 * __builtin_fmaf(r, Bp - c5, d);
-    e[8] = Lat<float>::w5 * __builtin_fmaf(r, Bm - c6, d);
+    const float r = 1.0f + d, t = mxx + myy;
+    const float w1d = w1 * d, w5d = w5 * d;
+    e[0] = __builtin_fmaf(r, (w0 * KA) * t, w0 * d);
+    const float o1 = (3.0f * w1) * r, o5 = (3.0f * w5) * r;
+    const float s13 = __builtin_fmaf(r, __builtin_fmaf(w1 * KB, mxx, (w1 * KA) * myy), w1d);
+    const float s24 = __builtin_fmaf(r, __builtin_fmaf(w1 * KA, mxx, (w1 * KB) * myy), w1d);
+    const float v1 = o1 * ux, v2 = o1 * uy;
+    e[1] = s13 + v1;
+    e[3] = s13 - v1;
+    e[2] = s24 + v2;
+    e[4] = s24 - v2;
+    const float b = (w5 * KB) * t, z = (9.0f * w5) * mxy;
+    const float s57 = __builtin_fmaf(r, b + z, w5d), s68 = __builtin_fmaf(r, b - z, w5d);
+    const float v5 = o5 * (ux + uy), v6 = o5 * (uy - ux);
+    e[5] = s57 + v5;
+    e[7] = s57 - v5;
+    e[6] = s68 + v6;
+    e[8] = s68 - v6;
 }
 
 __device__ __forceinline__ void moments_dev(const float e[Q], float &d, float &ux, float &uy, float &mxx,
                                             float &myy, float &mxy)
 {
-    constexpr float HA = Lat<float>::HA, HB = Lat<float>::HB;
+    constexpr float HA = Lat<float>::HA;
     const float a13 = e[1] + e[3], a24 = e[2] + e[4], a57 = e[5] + e[7], a68 = e[6] + e[8];
     const float diag = a57 + a68;
-    d = ((e[0] + (a13 + a24)) + diag) + DEV_MASS_F;
+    const float s0 = (e[0] + (a13 + a24)) + diag;
+    d = s0 + DEV_MASS_F;
     const float d57 = e[5] - e[7], d68 = e[6] - e[8]; // e8 - e6 == -d68 exactly
     const float sx = (e[1] - e[3]) + (d57 - d68);
     const float sy = (e[2] - e[4]) + (d57 + d68);
-    const float sxx = __builtin_fmaf(HA, e[0] + a24, HB * (a13 + diag));
-    const float syy = __builtin_fmaf(HA, e[0] + a13, HB * (a24 + diag));
+    // sum_k e_k (c_ka^2 - cs2) = (sum of e_k with c_ka != 0) - cs2 sum_k e_k
+    const float sxx = __builtin_fmaf(HA, s0, a13 + diag);
+    const float syy = __builtin_fmaf(HA, s0, a24 + diag);
     const float sxy = a57 - a68;
     const float inv = __builtin_amdgcn_rcpf(1.0f + d); // 1 ulp reciprocal
     ux = sx * inv;
