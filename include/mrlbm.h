@@ -69,9 +69,10 @@ typedef struct mrlbm_params {
                                      over the plain interior where four consecutive steps allow it
                                      (grids of at least 32 x 32; no plot step among them), else pairs;
                                      2: pairs only.  Same bits either way */
-    int32_t quad_rows;            /* four-step strip height: 0 (default) auto (20 rows; FP32 12, FP64 8
-                                     below 4096 waves), else FP32 12, 16, 20, 24, 28 or 32, FP64 8,
-                                     16, 20 or 24.  Same bits either way */
+    int32_t quad_rows;            /* four-step strip height: 0 (default) auto (FP32 20 rows, FP64 40,
+                                     each while it leaves 4096 waves, then 20, then FP32 12 / FP64
+                                     8), else FP32 12, 16, 20, 24, 28 or 32, FP64 8, 16, 20, 24, 32
+                                     or 40.  Same bits either way */
     int32_t reserved[2];
 } mrlbm_params;
 

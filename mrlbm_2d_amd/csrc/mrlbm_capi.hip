@@ -351,17 +351,19 @@ static int setup_time_blocking(mrlbm_handle *h, int H)
 // Four-step pass: sub-step s = 1..4 writes state t+s on the 7-s layers inside the ring and
 // the cylinder box dilated by 7-s (grown to the stencil nodes dilated by 4-s), into S2, S3, S2
 // and the destination; the four-step bulk writes the nodes four or more from the ring and
-// the box.  Strips of 20 rows (both forms; at 8192 x 2048 FP32 1.2% and FP64 2.6% faster per
-// step than the 24 and 16 rows of a sweep over 8..32), 12 (FP32) or 8 (FP64) rows when that
-// leaves fewer than 4096 waves, or the caller's quad_rows.
+// the box.  Strips of 20 rows in FP32 (at 8192 x 2048 1.2% faster per step than the 24 rows
+// of a sweep over 12..52), 40 in FP64 (5% faster than 20; a sweep over 8..48), each taken
+// while it leaves at least 4096 waves, then 20, then 12 (FP32) or 8 (FP64) rows; or the
+// caller's quad_rows.
 static int setup_four_step(mrlbm_handle *h, int quad_rows)
 {
     const int nx = h->nx, ny = h->ny;
     HIPCHK(hipMalloc(&h->S3, (size_t)6 * h->plane * (h->f32 ? sizeof(float) : sizeof(double))));
-    const int wide = 20, narrow = h->f32 ? 12 : 8;
+    const int tallest = h->f32 ? 20 : 40, narrow = h->f32 ? 12 : 8;
     h->ncgN = (nx - 8 + 55) / 56; // output columns 4..nx-5, 56 per wave
     h->ncbN = (h->ncgN + 3) / 4;
-    h->ms_H = (long long)h->ncgN * ((ny - 8 + wide - 1) / wide) >= 4096 ? wide : narrow;
+    auto waves = [&](int H) { return (long long)h->ncgN * ((ny - 8 + H - 1) / H); };
+    h->ms_H = waves(tallest) >= 4096 ? tallest : waves(20) >= 4096 ? 20 : narrow;
     if (quad_rows > 0) h->ms_H = quad_rows;
     h->nstripsN = (ny - 8 + h->ms_H - 1) / h->ms_H; // output rows 4..ny-5
     for (int s = 1; s <= 4; s++) h->nrectq[s - 1] = band_rects(h, 7 - s, 4 - s, h->rectq[s - 1]);
@@ -455,7 +457,7 @@ extern "C" int mrlbm_create(const mrlbm_params *p, mrlbm_handle **out)
             return fail(MRLBM_E_ARG, "steps_per_pass must be 0, 2 or 4");
         if (p->quad_rows != 0 && !(h->f32 ? bulkn_rows_ok<float>(p->quad_rows) : bulkn_rows_ok<double>(p->quad_rows)))
             return fail(MRLBM_E_ARG, h->f32 ? "quad_rows must be 0, 12, 16, 20, 24, 28 or 32 (FP32)"
-                                            : "quad_rows must be 0, 8, 16, 20 or 24 (FP64)");
+                                            : "quad_rows must be 0, 8, 16, 20, 24, 32 or 40 (FP64)");
         if (tb && h->nx >= 16 && h->ny >= 16) {
             const int rc3 = setup_time_blocking(h, tb);
             if (rc3) return fail(rc3, "");

@@ -1601,12 +1601,12 @@ static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t t0
     else hipLaunchKernelGGL(kernel, g, b, 0, st, args...);
 }
 
-// Four-step strip heights built: FP32 12..32 (multiples of 4), FP64 8, 16, 20, 24 (at 12 rows
-// the compiler moves the FP64 march's carried rows to scratch memory).
+// Four-step strip heights built: FP32 12..32 (multiples of 4), FP64 8, 16, 20, 24, 32, 40 (at
+// 12 and 52 rows the compiler moves the FP64 march's carried rows to scratch memory).
 template <class S> bool bulkn_rows_ok(int H)
 {
     if constexpr (sizeof(S) == 4) return H == 12 || H == 16 || H == 20 || H == 24 || H == 28 || H == 32;
-    else return H == 8 || H == 16 || H == 20 || H == 24;
+    else return H == 8 || H == 16 || H == 20 || H == 24 || H == 32 || H == 40;
 }
 // calls f with std::integral_constant<int, H> for a built strip height H (false otherwise)
 template <class S, class F> static bool with_bulkn_rows(int H, F &&f)
@@ -1626,6 +1626,8 @@ template <class S, class F> static bool with_bulkn_rows(int H, F &&f)
         case 16: f(std::integral_constant<int, 16>{}); return true;
         case 20: f(std::integral_constant<int, 20>{}); return true;
         case 24: f(std::integral_constant<int, 24>{}); return true;
+        case 32: f(std::integral_constant<int, 32>{}); return true;
+        case 40: f(std::integral_constant<int, 40>{}); return true;
         }
     }
     return false;

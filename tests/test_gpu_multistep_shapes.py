@@ -10,7 +10,7 @@ from mrlbm_2d_amd import capi
 
 pytestmark = pytest.mark.gpu
 
-HEIGHTS = {capi.MRLBM_FP32_DEV: (12, 16, 20, 24, 28, 32), capi.MRLBM_FP64: (8, 16, 20, 24)}
+HEIGHTS = {capi.MRLBM_FP32_DEV: (12, 16, 20, 24, 28, 32), capi.MRLBM_FP64: (8, 16, 20, 24, 32, 40)}
 
 
 def _params(precision, nx, ny, xc, yc, rc, re=100.0, umax=0.05):
