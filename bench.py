@@ -16,7 +16,8 @@ committed profile named in `traffic_source`, not measured in this run; below the
 bytes because the intermediate steps never leave the chip, so `frac` (algorithmic) passes 1
 and `traffic_frac` (those bytes over the same launch time, against 8 TB/s) is the share of HBM
 bandwidth the launch actually uses; `valu_issue` (same profile) is the share of the SIMDs'
-cycles the four-step launch spends issuing VALU instructions, the bound it runs into.
+cycles the four-step launch spends issuing VALU instructions (instruction issue, not HBM,
+bounds that kernel).
 `statistics_window`: ms per step inside the statistics window (forces, surface pressure,
 probes and u statistics recorded every step) against plain steps, both forms.
 `fp32_vs_fp64` is the accuracy condition of the FP32 figure: both forms from the same seeded
@@ -44,16 +45,16 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel at 8192 x 2048, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a plain copy of known size (FETCH_SIZE x 2, WRITE_SIZE x 1):
-# the four-step bulk pass (default; strips of 20 rows in both forms) and the two-step one
-# (--steps-per-pass 2; FP32 10 rows, FP64 12), with the profiles holding the summaries and
+# the four-step bulk pass (default; strips of 20 rows in FP32, 40 in FP64) and the two-step
+# one (--steps-per-pass 2; FP32 10 rows, FP64 12), with the profiles holding the summaries and
 # the commands; the strip heights each was measured at
 TRAFFIC = {
-    4: ({"f32": 1_070_583_778, "f64": 2_017_994_926}, "profiles/bulkn_kernel_8192x2048_hbm_bytes.md", (20, 20)),
+    4: ({"f32": 1_074_217_287, "f64": 1_851_616_351}, "profiles/bulkn_kernel_8192x2048_hbm_bytes.md", (20, 40)),
     2: ({"f32": 949_550_578, "f64": 2_289_678_504}, "profiles/bulk2_kernel_8192x2048_hbm_bytes.md", (10, 12)),
 }
-# Share of the SIMDs' cycles issuing VALU instructions during a four-step launch (the bound of
-# that kernel), from the same profile's SQ_INSTS_VALU x 4 cycles over GRBM_GUI_ACTIVE
-VALU_ISSUE = {4: ({"f32": 0.918, "f64": 0.848}, "profiles/bulkn_kernel_8192x2048_hbm_bytes.md", (20, 20))}
+# Share of the SIMDs' cycles issuing VALU instructions during a four-step launch (instruction
+# issue bounds that kernel), from the same profile's SQ_INSTS_VALU x 4 cycles over GRBM_GUI_ACTIVE
+VALU_ISSUE = {4: ({"f32": 0.805, "f64": 0.743}, "profiles/bulkn_kernel_8192x2048_hbm_bytes.md", (20, 40))}
 NX, NY = 8192, 2048
 # HIP event pairs around every 4th bulk launch on long runs: the per-launch mean without the
 # events' own cost (about 2% of the step when recorded around every launch); runs of up to

@@ -42,9 +42,10 @@ struct mrlbm_batch {
     void *d_q[4][2][2] = {};                                   // sub-steps 1..4 [parity][record]
     int2 *d_stiles[3] = {nullptr, nullptr, nullptr}; // side blocks (run, block): single step, sub-step 1, 2
     int nstiles[3] = {0, 0, 0};
-    int2 *d_wtiles = nullptr;     // wall blocks (run, block of 64 boundary nodes)
-    int nwtiles = 0;
+    int2 *d_wtiles = nullptr;     // wall blocks (run, block of 64 boundary nodes) of the runs
+    int nwtiles = 0;              // whose walls are not in wall groups
     bool wall_levels = false;     // some wall has more than one dependency level
+    bool walls = false;           // some run has a curved wall
     int64_t nodes = 0;
     hipStream_t st = nullptr, st2 = nullptr;
     hipEvent_t evA = nullptr, evB = nullptr, e0 = nullptr, e1 = nullptr;
@@ -106,29 +107,37 @@ template <class S, class T> static int build_tables(mrlbm_batch *b)
     std::vector<StepArgs<S, T>> args[2][2], b2[2], s1[2][2], s2[2][2], b4[2], q[4][2][2];
     std::vector<int2> tiles, tiles2, tiles4;
     std::vector<int2> stiles[3], wtiles, qtiles[4];
-    b->wall_levels = false;
+    b->wall_levels = b->walls = false;
+    for (int r = 0; r < R; r++) {
+        b->walls = b->walls || b->runs[r]->has_b;
+        b->wall_levels = b->wall_levels || (b->runs[r]->has_b && b->runs[r]->bt.nlevels > 1);
+    }
     auto add_blocks = [](std::vector<int2> &v, int r, int n) {
         for (int k = 0; k < (n + 63) / 64; k++) v.push_back(make_int2(r, k));
     };
     for (int r = 0; r < R; r++) {
         mrlbm_handle *h = b->runs[r];
-        // side blocks: the marched band rows, the ring's column tiles, then 64 items each (ring
-        // rows or the whole ring, the other bands, stencil nodes), as side_block counts them
+        // A wall of one level in wall groups runs inside the side blocks, as in a single run
+        // (then no stencil items and no wall blocks); with some wall of several levels every
+        // wall goes to the per-run boundary workgroups.
+        const int nwg = (h->has_b && !b->wall_levels) ? h->bd.nwg : 0;
+        // side blocks: the wall groups, the marched band rows, the ring's column tiles, then 64
+        // items each (ring rows or the whole ring, the other bands, stencil nodes), as
+        // side_block counts them
         const int nct = ring_column_tiles(h->nx, h->ny, false);
-        const int ring = 64 * nct + (nct ? 2 * h->nx : 2 * h->nx + 2 * (h->ny - 2)), nU = h->has_b ? h->bt.nU : 0;
+        const int ring = 64 * (nwg + nct) + (nct ? 2 * h->nx : 2 * h->nx + 2 * (h->ny - 2));
+        const int nU = h->has_b && nwg == 0 ? h->bt.nU : 0;
         auto band = [&](const int rc[5][4], int n) {
             const int bm = band_march_rects(rc, n);
             return (int)(64 * band_march_blocks(rc, n, bm) + band_items(rc, n, bm));
         };
         add_blocks(stiles[0], r, ring + nU);
-        if (h->has_b) {
-            add_blocks(wtiles, r, h->bt.nbct);
-            b->wall_levels = b->wall_levels || h->bt.nlevels > 1;
-        }
+        if (h->has_b && nwg == 0) add_blocks(wtiles, r, h->bt.nbct);
         for (int par = 0; par < 2; par++)
             for (int rec = 0; rec < 2; rec++) {
                 auto a = base_args<S, T>(h);
                 a.pops = nullptr;
+                a.wall_groups = nwg;
                 a.rec_slot = rec ? 0 : -1;
                 a.facc = rec ? b->d_facc + 4 * r : nullptr;
                 auto one = a;
@@ -231,7 +240,7 @@ template <class S, class T> static int launch_step(mrlbm_batch *b, int par, int 
     BHIP(launch_bulk_batch<S, T>(args, b->d_tiles, b->ntiles, b->runs[0]->variant, b->grid, b->st));
     BHIP(hipStreamWaitEvent(b->st2, b->evA, 0));
     BHIP(launch_side_tiles<S, T>(args, b->d_stiles[0], b->nstiles[0], b->d_wtiles, b->nwtiles, R, b->wall_levels,
-                                 rec != 0, b->st2));
+                                 b->walls, rec != 0, b->st2));
     BHIP(hipEventRecord(b->evB, b->st2));
     BHIP(hipEventRecord(b->evA, b->st));
     return 0;
@@ -246,9 +255,9 @@ template <class S, class T> static int launch_pair(mrlbm_batch *b, int par, int 
     BHIP(launch_bulk2_batch<S, T>((const StepArgs<S, T> *)b->d_b2[par], b->d_tiles2, b->ntiles2, b->tb, b->st));
     BHIP(hipStreamWaitEvent(b->st2, b->evA, 0));
     BHIP(launch_side_tiles<S, T>((const StepArgs<S, T> *)b->d_s1[par][rec], b->d_stiles[1], b->nstiles[1], b->d_wtiles,
-                                 b->nwtiles, R, b->wall_levels, rec != 0, b->st2));
+                                 b->nwtiles, R, b->wall_levels, b->walls, rec != 0, b->st2));
     BHIP(launch_side_tiles<S, T>((const StepArgs<S, T> *)b->d_s2[par][rec], b->d_stiles[2], b->nstiles[2], b->d_wtiles,
-                                 b->nwtiles, R, b->wall_levels, rec != 0, b->st2));
+                                 b->nwtiles, R, b->wall_levels, b->walls, rec != 0, b->st2));
     BHIP(hipEventRecord(b->evB, b->st2));
     BHIP(hipEventRecord(b->evA, b->st));
     return 0;
@@ -264,7 +273,7 @@ template <class S, class T> static int launch_quad(mrlbm_batch *b, int par, int 
     BHIP(hipStreamWaitEvent(b->st2, b->evA, 0));
     for (int k = 0; k < 4; k++)
         BHIP(launch_side_tiles<S, T>((const StepArgs<S, T> *)b->d_q[k][par][rec], b->d_qtiles[k], b->nqtiles[k],
-                                     b->d_wtiles, b->nwtiles, R, b->wall_levels, rec != 0, b->st2));
+                                     b->d_wtiles, b->nwtiles, R, b->wall_levels, b->walls, rec != 0, b->st2));
     BHIP(hipEventRecord(b->evB, b->st2));
     BHIP(hipEventRecord(b->evA, b->st));
     return 0;

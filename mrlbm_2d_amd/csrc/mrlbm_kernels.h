@@ -188,12 +188,13 @@ hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st);
 template <class S, class T>
 hipError_t launch_bulk_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int variant, int grid,
                              hipStream_t st);
-// Side work of every run of a batch (single-wave blocks from the (run, block) tables), then
-// the walls: per-run workgroups when some wall has several dependency levels, else wall
-// blocks plus, on recording steps, one records workgroup per run.
+// Side work of every run of a batch (single-wave blocks from the (run, block) tables; a wall
+// of one level in wall groups is among them), then the walls: per-run workgroups when some
+// wall has several dependency levels, else the wall blocks of the walls outside wall groups
+// plus, on recording steps, one records workgroup per run.
 template <class S, class T>
 hipError_t launch_side_tiles(const StepArgs<S, T> *args, const int2 *stiles, int nstiles, const int2 *wtiles,
-                             int nwtiles, int nruns, bool wall_levels, bool records, hipStream_t st);
+                             int nwtiles, int nruns, bool wall_levels, bool walls, bool records, hipStream_t st);
 template <class S, class T>
 hipError_t launch_bulk2_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int H, hipStream_t st);
 // Four steps per pass for every run of a batch (tiles: (run, four-step block)); strips of H

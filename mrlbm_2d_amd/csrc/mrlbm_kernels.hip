@@ -1287,13 +1287,14 @@ template <class S, class T> __global__ void __launch_bounds__(256) boundary_batc
     boundary_body(a);
 }
 
-// Side work of every run from a compact table of (run, block of 64 side nodes): ring, band
-// rectangles and stencil nodes in single-wave workgroups, none of them empty.
+// Side work of every run from a compact table of (run, block): wall groups, band marches, ring
+// column tiles, then blocks of 64 ring, band and stencil nodes, in single-wave workgroups,
+// none of them empty (side_block's order, per run).
 template <class S, class T> __global__ void __launch_bounds__(64) side_tiles_kernel(const StepArgs<S, T> *args, const int2 *tiles)
 {
     const int2 t = tiles[blockIdx.x];
     const StepArgs<S, T> &a = args[t.x];
-    side_block(a, t.y, band_nodes(a));
+    side_block<S, T, true>(a, t.y, band_nodes(a));
 }
 
 // Wall nodes of every run from a table of (run, block of 64 boundary nodes), for walls of one
@@ -1700,13 +1701,13 @@ template <class S, class T> int bulk_batch_resident_blocks(int variant)
 }
 template <class S, class T>
 hipError_t launch_side_tiles(const StepArgs<S, T> *args, const int2 *stiles, int nstiles, const int2 *wtiles,
-                             int nwtiles, int nruns, bool wall_levels, bool records, hipStream_t st)
+                             int nwtiles, int nruns, bool wall_levels, bool walls, bool records, hipStream_t st)
 {
     if (nstiles > 0) hipLaunchKernelGGL((side_tiles_kernel<S, T>), dim3(nstiles), dim3(64), 0, st, args, stiles);
     if (wall_levels) { // some wall interpolates in place by level: one workgroup per run
         hipLaunchKernelGGL((boundary_batch_kernel<S, T>), dim3(nruns), dim3(256), 0, st, args);
-    } else if (nwtiles > 0) {
-        hipLaunchKernelGGL((wall_tiles_kernel<S, T>), dim3(nwtiles), dim3(64), 0, st, args, wtiles);
+    } else if (walls) { // the walls outside wall groups, then the records of every wall
+        if (nwtiles > 0) hipLaunchKernelGGL((wall_tiles_kernel<S, T>), dim3(nwtiles), dim3(64), 0, st, args, wtiles);
         if (records) hipLaunchKernelGGL((wall_records_batch_kernel<S, T>), dim3(nruns), dim3(256), 0, st, args);
     }
     return hipGetLastError();
@@ -1905,7 +1906,7 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_wall<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
     template hipError_t launch_bulk_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, int, hipStream_t); \
     template hipError_t launch_side_tiles<S, T>(const StepArgs<S, T> *, const int2 *, int, const int2 *, int, int, bool, \
-                                                bool, hipStream_t);                                  \
+                                                bool, bool, hipStream_t);                            \
     template hipError_t launch_bulk2_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, hipStream_t); \
     template hipError_t launch_bulkn_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, hipStream_t); \
     template bool bulkn_rows_ok<S>(int);                                                         \
