@@ -6,11 +6,13 @@ velocity noise.  Both precision forms run: FP32 deviation storage (the headline 
 and plain FP64.  A "step" is one lattice update of every node; the default path advances
 four steps per launch of the bulk kernel (temporal blocking), bit-identical to single steps.
 
-Prints ONE JSON line.  `roofline` prices the dominant kernel against MI355X's 8 TB/s HBM:
-achieved = algorithmic bytes of the timed launches (12 moment words per node and step: 6
-read + 6 written; the library reports how many steps the timed launches advanced) / their
-summed duration, from a HIP event pair given to each timed launch on the bulk kernel's own
-stream (every launch when the run is short, every 4th otherwise);
+Prints ONE JSON line.  `value` is the wall clock of the K timed steps, synchronized on both
+sides, with no per-launch events among them.  `roofline` prices the dominant kernel against
+MI355X's 8 TB/s HBM over the next K steps of the same run: achieved = algorithmic bytes of
+the timed launches (12 moment words per node and step: 6 read + 6 written; the library
+reports how many steps the timed launches advanced) / their summed duration, from a HIP
+event pair given to each timed launch on the bulk kernel's own stream (every launch when the
+run is short, every 4th otherwise);
 traffic = one launch's HBM bytes from rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE) of the
 committed profile named in `traffic_source`, not measured in this run; below the algorithmic
 bytes because the intermediate steps never leave the chip, so `frac` (algorithmic) passes 1
@@ -119,12 +121,20 @@ def run_form(precision, steps, warmup, gate_steps=0, gate_window=0, **kw):
                       flush=True)
     lat.step(it, warmup)
     it += warmup
+    first_timed = it
+    # the timed steps run without per-launch events (a pair on every launch of a 20-step
+    # window costs ~3% of its wall clock); the next `steps` steps of the same run carry them,
+    # for the launch durations of the roofline
     lat.synchronize()
-    lat.set_timing(1 if steps <= TIME_ALL_UP_TO else TIMED_EVERY)
+    lat.set_timing(0)
     t0 = time.perf_counter()
     lat.step(it, steps)
     lat.synchronize()
     wall = time.perf_counter() - t0
+    it += steps
+    lat.set_timing(1 if steps <= TIME_ALL_UP_TO else TIMED_EVERY)
+    lat.step(it, steps)
+    lat.synchronize()
     t = lat.last_timing()
     finite = lat.check_finite() == 0
     assert finite, "non-finite state after the timed steps"
@@ -145,7 +155,7 @@ def run_form(precision, steps, warmup, gate_steps=0, gate_window=0, **kw):
                 launch_max_ms=t.bulk_max_ms, timed_launches=t.timed_launches, bulk_launches=t.bulk_launches,
                 steps_per_launch=t.bulk_steps / max(t.bulk_launches, 1), timed_steps=t.timed_steps,
                 bytes_per_step=bytes_per_step, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
-                first_timed_step=it, pair_rows=info.pair_rows, quad_rows=info.quad_rows), out
+                first_timed_step=first_timed, pair_rows=info.pair_rows, quad_rows=info.quad_rows), out
 
 
 def roofline(r, kernel, traffic, source, valu=None):
