@@ -83,23 +83,28 @@ template <class S, class T> struct StepArgs {
 // The band rectangles of a multi-step sub-step: 0 and 1 the rows next to the channel walls
 // (rows 1..L and ny-1-L..ny-2, columns 1..nx-2), 2 and 3 the columns next to the inlet and
 // outlet (L wide), 4 the cylinder's rectangle.  The side launch marches 0, 1 and 4 row by
-// row (62 columns and up to BAND_STRIP rows per single-wave workgroup) and takes the narrow
-// 2 and 3 node by node.  The marched rectangles (a bit mask), the workgroups rectangle q's
-// march takes, all marched workgroups, and the nodes taken node by node.
-constexpr int BAND_STRIP = 16;
-__host__ __device__ inline int band_march_rects(const int (*)[4], int n)
+// row (62 columns and up to BAND_STRIP rows per single-wave workgroup), and 2 and 3 (when at
+// most 6 columns wide) row by row in packed segments: 8 segments of 8 lanes per wave, each
+// the L columns and their two halo columns of its own BAND_SEG_ROWS rows.  The marched
+// rectangles (a bit mask), the workgroups rectangle q's march takes, all marched workgroups,
+// and the nodes taken node by node.
+constexpr int BAND_STRIP = 16, BAND_SEG_ROWS = 16;
+__host__ __device__ constexpr bool band_column_rect(int q) { return q == 2 || q == 3; }
+__host__ __device__ inline int band_march_rects(const int (*rc)[4], int n)
 {
-    return (n >= 2 ? 3 : 0) | (n >= 5 ? 16 : 0);
+    const int cols = n >= 4 && rc[2][2] <= 6 && rc[3][2] <= 6 ? 12 : 0;
+    return (n >= 2 ? 3 : 0) | cols | (n >= 5 ? 16 : 0);
 }
-__host__ __device__ inline int band_march_rect_blocks(const int rc[4])
+__host__ __device__ inline int band_march_rect_blocks(const int rc[4], int q)
 {
+    if (band_column_rect(q)) return (rc[3] + 8 * BAND_SEG_ROWS - 1) / (8 * BAND_SEG_ROWS);
     return ((rc[2] + 61) / 62) * ((rc[3] + BAND_STRIP - 1) / BAND_STRIP);
 }
 __host__ __device__ inline int band_march_blocks(const int rc[][4], int n, int marched)
 {
     int b = 0;
     for (int q = 0; q < n; q++)
-        if ((marched >> q) & 1) b += band_march_rect_blocks(rc[q]);
+        if ((marched >> q) & 1) b += band_march_rect_blocks(rc[q], q);
     return b;
 }
 __host__ __device__ inline long long band_items(const int rc[][4], int n, int marched)

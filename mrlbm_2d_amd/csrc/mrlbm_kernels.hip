@@ -790,15 +790,20 @@ template <class S, class T> __device__ __forceinline__ int band_nodes(const Step
 // each row is the bulk march's single-step row update (march_row) from rows loaded one
 // ahead.  Node classes come from the cylinder's box (boundary nodes are the wall's, solid
 // nodes are zeroed on plot steps); nodes of an earlier rectangle are left to it.
+template <class S, class T> __device__ void band_march_segments(const StepArgs<S, T> &a, int q, int blk);
 template <class S, class T> __device__ void band_march(const StepArgs<S, T> &a, int blk)
 {
     using F = typename FormOf<T>::F;
     int q = 0;
     for (; q < a.nrect; q++) { // the rectangle of this workgroup
         if (!((a.band_march >> q) & 1)) continue;
-        const int nb = band_march_rect_blocks(a.rect[q]);
+        const int nb = band_march_rect_blocks(a.rect[q], q);
         if (blk < nb) break;
         blk -= nb;
+    }
+    if (band_column_rect(q)) {
+        band_march_segments(a, q, blk);
+        return;
     }
     const int x0 = a.rect[q][0], w = a.rect[q][2], per = (w + 61) / 62;
     const int g = blk % per, st = blk / per;
@@ -825,6 +830,48 @@ template <class S, class T> __device__ void band_march(const StepArgs<S, T> &a, 
         march_row<S, T, true>(a, c, valid(j), j, mA, up_prev, up_cur, mid_cur, cls_of(j));
         if (j + 1 >= y1) break;
         if (j + 3 <= y1) load_moments(a, cl, j + 3, mA);
+        march_row<S, T, true>(a, c, valid(j + 1), j + 1, mB, up_prev, up_cur, mid_cur, cls_of(j + 1));
+    }
+}
+
+// Marched narrow column rectangle q (at most 6 columns, next to the inlet or outlet): the
+// wave's lanes in 8 segments of 8, segment s of workgroup blk taking rows y0 + (8 blk + s) R
+// .. + R-1 (R = BAND_SEG_ROWS) of the rectangle; within a segment, lane i holds column
+// x0 - 1 + i (lanes 1..w write, lanes 0 and w+1 carry the halo columns).  Each segment
+// marches its rows with march_row from rows loaded one ahead, as band_march does.  The
+// whole-wave lane shifts hand the halo lanes values of the neighbouring segment, which only
+// reach the halo lanes' own (unwritten) nodes.  Node classes from the cylinder's box; nodes
+// of an earlier rectangle are left to it.
+template <class S, class T> __device__ void band_march_segments(const StepArgs<S, T> &a, int q, int blk)
+{
+    using F = typename FormOf<T>::F;
+    constexpr int R = BAND_SEG_ROWS;
+    const int x0 = a.rect[q][0], w = a.rect[q][2], yr = a.rect[q][1], h = a.rect[q][3];
+    const int lane = threadIdx.x, seg = lane >> 3, i = lane & 7;
+    const int c = x0 - 1 + i, cl = min(c, a.nx - 1); // lanes past the halo load a column again
+    const int y0 = yr + (8 * blk + seg) * R, y1 = min(y0 + R, yr + h); // rows y0 .. y1-1
+    const bool lanev = i >= 1 && i <= w;
+    auto row = [&](int j) { return min(j, a.ny - 1); }; // segments past the rectangle end
+    auto cls_of = [&](int j) {
+        const int bi = c - a.bx0, bj = j - a.by0;
+        return ((unsigned)bi < (unsigned)a.bw && (unsigned)bj < (unsigned)a.bh) ? (int)a.boxcls[bj * a.bw + bi] : 0;
+    };
+    auto valid = [&](int j) { return lanev && j < y1 && !in_rects(a, q, c, j); };
+    S mA[6], mB[6];
+    T F9[Q], up_prev[3], up_cur[3], mid_cur[3];
+    load_moments(a, cl, row(y0 - 1), mA);
+    F::recon(mA, F9);
+    up_prev[0] = F9[2]; up_prev[1] = F9[5]; up_prev[2] = F9[6];
+    load_moments(a, cl, row(y0), mA);
+    F::recon(mA, F9);
+    up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
+    mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
+    load_moments(a, cl, row(y0 + 1), mA);
+    for (int k = 0; k < R; k += 2) { // every segment marches R rows (uniform trip count)
+        const int j = y0 + k;
+        load_moments(a, cl, row(j + 2), mB);
+        march_row<S, T, true>(a, c, valid(j), j, mA, up_prev, up_cur, mid_cur, cls_of(j));
+        if (k + 3 <= R) load_moments(a, cl, row(j + 3), mA);
         march_row<S, T, true>(a, c, valid(j + 1), j + 1, mB, up_prev, up_cur, mid_cur, cls_of(j + 1));
     }
 }
