@@ -141,7 +141,7 @@ template <class S, class T> static int create_typed(mrlbm_handle *h)
     }
     HIPCHK(hipMalloc(&h->uin, h->ny * sizeof(T)));
     HIPCHK(hipMemcpy(h->uin, uin.data(), h->ny * sizeof(T), hipMemcpyHostToDevice));
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < kInletRing; b++) {
         h->d_inlet[b] = dmalloc<double>(h->ny, e);
         HIPCHK(e);
     }
@@ -204,10 +204,10 @@ template <class S, class T> static int create_typed(mrlbm_handle *h)
         b.ub = dmalloc<T>((size_t)2 * t.nbct, e);
         HIPCHK(e);
         h->allocs.push_back(b.ub);
-        b.pcy = dmalloc<double>(t.nbct, e);
+        b.pcy = dmalloc<double>((size_t)kWallSlots * t.nbct, e); // slot 0 unless in a four-step pass
         HIPCHK(e);
         h->allocs.push_back(b.pcy);
-        b.fxy = dmalloc<double>((size_t)2 * t.nbct, e);
+        b.fxy = dmalloc<double>((size_t)kWallSlots * 2 * t.nbct, e);
         HIPCHK(e);
         h->allocs.push_back(b.fxy);
         // the surface-pressure smoothing weights (nbct^2 doubles; up to 1 GiB, else computed
@@ -231,8 +231,10 @@ template <class S, class T> static int create_typed(mrlbm_handle *h)
 static void free_all(mrlbm_handle *h)
 {
     adjoint_free(h);
-    for (void *p : {h->S[0], h->S[1], h->pops, h->ghost, h->uin, (void *)h->d_boxbits, (void *)h->d_rec, (void *)h->d_inlet[0],
-                    (void *)h->d_inlet[1], (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red, h->chi, h->S2, h->S3, (void *)h->d_dense})
+    for (void *p : {h->S[0], h->S[1], h->pops, h->ghost, h->uin, (void *)h->d_boxbits, (void *)h->d_rec,
+                    (void *)h->d_rho_mean, (void *)h->d_acc, (void *)h->d_red, h->chi, h->S2, h->S3, (void *)h->d_dense})
+        if (p) (void)hipFree(p);
+    for (double *p : h->d_inlet)
         if (p) (void)hipFree(p);
     for (void *p : h->allocs)
         if (p) (void)hipFree(p);
@@ -716,7 +718,7 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
         a.record_p = window && opts.record_surface_p ? 1 : 0;
         a.nprobe = opts.nprobe;
         for (int p = 0; p < 4; p++) a.probe_node[p] = probe_node[p];
-        a.inlet_out = window ? h->d_inlet[iter & 1] : nullptr;
+        a.inlet_out = window ? h->d_inlet[iter & (kInletRing - 1)] : nullptr;
         a.inlet_prev = (h->prev_inlet_valid && h->has_b) ? h->d_inlet[h->prev_inlet_buf] : nullptr;
         a.inlet_prev_c = h->prev_inlet_c;
         a.wall_groups = h->bd.nwg; // the wall nodes as wall groups inside the side launch
@@ -727,10 +729,10 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
     // the inlet-density fold.  The wall chain used to run on a third stream beside the ring
     // kernel; every launch beside the bulk costs it time, and the fork and join with their
     // extra launch cost more than the overlap gained (8192 x 2048: 2.2% FP32, 1% FP64).
-    auto side_work = [&](const StepArgs<S, T> &a) -> hipError_t {
+    auto side_work = [&](const StepArgs<S, T> &a, bool records = true) -> hipError_t {
         hipError_t e = launch_side<S, T>(a, h->st2, a.wall_groups == 0);
         if (e != hipSuccess) return e;
-        if (h->has_b) return launch_wall<S, T>(a, h->st2);
+        if (h->has_b) return launch_wall<S, T>(a, h->st2, records);
         if (h->prev_inlet_valid) {
             auto f = a;
             f.inlet_prev = h->d_inlet[h->prev_inlet_buf];
@@ -742,7 +744,7 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
     auto after_iter = [&](int iter) {
         const bool window = is_window(iter);
         h->prev_inlet_valid = window ? 1 : 0;
-        h->prev_inlet_buf = iter & 1;
+        h->prev_inlet_buf = iter & (kInletRing - 1);
         h->prev_inlet_c = (double)(float)(iter - opts.statsbegin); // real() is REAL(4) (:358)
         if (sample_step(iter)) h->nsamples++;
     };
@@ -859,6 +861,12 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
                 h->bulk_done = h->evA;
             }
             int slot = slot0;
+            // a wall of one level records after the pass's side work: sub-step k's wall writes
+            // slot k of the per-node force and pressure arrays, and one launch then takes the
+            // records of the four steps in order (the inlet densities wait in their ring)
+            const bool defer = h->has_b && h->bd.nlevels <= 1;
+            RecSteps rs{};
+            StepArgs<S, T> last{};
             for (int k = 0; k < 4; k++) {
                 auto a = step_args(iter + k, src[k], dst[k], slot);
                 if (is_window(iter + k)) slot++;
@@ -866,8 +874,26 @@ template <class S, class T> static int step_typed(mrlbm_handle *h, int iter_firs
                 a.nrect = h->nrectq[k];
                 std::memcpy(a.rect, h->rectq[k], sizeof h->rectq[k]);
                 a.band_march = band_march_rects(a.rect, a.nrect);
-                HIPCHK(side_work(a));
+                if (defer) {
+                    a.b.fxy = h->bd.fxy + (size_t)k * 2 * h->bd.nbct;
+                    a.b.pcy = h->bd.pcy + (size_t)k * h->bd.nbct;
+                    if (a.rec_slot >= 0 || a.inlet_prev) {
+                        rs.rec_slot[rs.n] = a.rec_slot;
+                        rs.record_p[rs.n] = a.record_p;
+                        rs.wslot[rs.n] = k;
+                        rs.inlet_prev[rs.n] = a.inlet_prev;
+                        rs.inlet_prev_c[rs.n] = a.inlet_prev_c;
+                        rs.n++;
+                    }
+                    last = a;
+                }
+                HIPCHK(side_work(a, !defer));
                 after_iter(iter + k);
+            }
+            if (defer && rs.n > 0) {
+                last.b.fxy = h->bd.fxy; // the records kernels add the slot offsets
+                last.b.pcy = h->bd.pcy;
+                HIPCHK(launch_records4<S, T>(last, rs, h->st2));
             }
             HIPCHK(hipEventRecord(h->evB, h->st2));
             h->cur ^= 1;

@@ -28,6 +28,12 @@ template <class T> inline T *dmalloc(size_t n, hipError_t &e)
 constexpr int kRedWords = 4096;
 // u statistics accumulators (FP64 planes): sum ux, sum uy, sum ux^2, sum uy^2
 constexpr int kAccPlanes = 4;
+// Inlet-density buffers of the window steps (by iteration modulo the count): a four-step
+// pass folds them into the running mean after its side work, so five must be held at once.
+constexpr int kInletRing = 8;
+static_assert((kInletRing & (kInletRing - 1)) == 0, "the ring index is iter & (kInletRing - 1)");
+// Slots of the wall's per-node force and pressure arrays: one per step of a four-step pass.
+constexpr int kWallSlots = 4;
 
 struct mrlbm_handle {
     mrlbm_params p{};
@@ -65,7 +71,7 @@ struct mrlbm_handle {
     BoundaryDev bd{};
     double *d_rec = nullptr;
     int rec_cap = 0, rec_stride = 0;
-    double *d_inlet[2] = {nullptr, nullptr};
+    double *d_inlet[kInletRing] = {}; // inlet densities of window steps, by iter % kInletRing
     double *d_rho_mean = nullptr;
     double *d_acc = nullptr;
     int64_t nsamples = 0;

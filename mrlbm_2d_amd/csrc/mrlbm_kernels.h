@@ -188,7 +188,18 @@ __host__ __device__ constexpr int ring_column_tiles(int nx, int ny, bool from_po
 template <class S, class T>
 hipError_t launch_side(const StepArgs<S, T> &a, hipStream_t st, bool stencil);
 template <class S, class T>
-hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st);
+hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st, bool records = true);
+// The steps of a four-step pass whose records run after the pass's side work (one workgroup
+// for all of them, then one smoothing launch): per step its record slot, whether it records
+// surface pressure, the slot of the per-node force and pressure arrays its wall wrote, and
+// the inlet densities of the step before it (with that step's count) to fold into the mean.
+struct RecSteps {
+    int n;
+    int rec_slot[4], record_p[4], wslot[4];
+    const double *inlet_prev[4];
+    double inlet_prev_c[4];
+};
+template <class S, class T> hipError_t launch_records4(const StepArgs<S, T> &a, const RecSteps &r, hipStream_t st);
 // Batched runs: args[r] per run in device memory; tiles[t] = (run, bulk block of that run).
 template <class S, class T>
 hipError_t launch_bulk_batch(const StepArgs<S, T> *args, const int2 *tiles, int ntiles, int variant, int grid,

@@ -1187,14 +1187,15 @@ __global__ void __launch_bounds__(256) smooth_weights_kernel(BoundaryDev b, doub
 // partial sums, so the value does not depend on which wave computes it.  The weight sum is
 // the precomputed one (the reference's sequential order) when the weights are stored.
 // The reference sums all nbct terms in order; this differs from it by rounding only.
-template <class S, class T> __device__ void smooth_output(const StepArgs<S, T> &a, double *rec, int t)
+template <class S, class T>
+__device__ void smooth_output(const StepArgs<S, T> &a, const double *pcy, double *rec, int t)
 {
     const BoundaryDev &b = a.b;
     const int lane = threadIdx.x & 63, n = b.nbct;
     double sv = 0.0, sw = 0.0;
     for (int q = lane; q < n; q += 64) {
         const double w = b.gw ? b.gw[(long long)q * n + t] : smooth_weight(b.thetas[t], b.thetas[q]);
-        sv = sv + w * b.pcy[b.idx[q]];
+        sv = sv + w * pcy[b.idx[q]];
         sw = sw + w;
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -1210,25 +1211,39 @@ template <class S, class T> __device__ void smooth_output(const StepArgs<S, T> &
 template <class S, class T> __global__ void __launch_bounds__(64) smooth_kernel(StepArgs<S, T> a)
 {
     double *rec = a.rec + (long long)a.rec_slot * a.rec_stride;
-    for (int t = blockIdx.x; t < a.b.nbct; t += gridDim.x) smooth_output(a, rec, t);
+    for (int t = blockIdx.x; t < a.b.nbct; t += gridDim.x) smooth_output(a, a.b.pcy, rec, t);
+}
+
+// The smoothing of the recording steps of a four-step pass (blockIdx.y: the step), after its
+// records kernel: step k's pressures are in slot r.wslot[k] of the per-node array.
+template <class S, class T> __global__ void __launch_bounds__(64) smooth4_kernel(StepArgs<S, T> a, RecSteps r)
+{
+    const int k = blockIdx.y;
+    if (!(r.record_p[k] && r.rec_slot[k] >= 0)) return;
+    double *rec = a.rec + (long long)r.rec_slot[k] * a.rec_stride;
+    const double *pcy = a.b.pcy + (long long)r.wslot[k] * a.b.nbct;
+    for (int t = blockIdx.x; t < a.b.nbct; t += gridDim.x) smooth_output(a, pcy, rec, t);
 }
 
 // Net force, records and the inlet-density running mean of one step (one workgroup), after
 // every boundary node of the step has been written.
-template <class S, class T> __device__ void wall_records(const StepArgs<S, T> &a)
+template <class S, class T>
+__device__ void wall_records_step(const StepArgs<S, T> &a, int rec_slot, int record_p, int smooth_apart,
+                                  const double *fxy, const double *pcy, const double *inlet_prev,
+                                  double inlet_prev_c)
 {
     const BoundaryDev &b = a.b;
     __shared__ double sh[16];
     const int tid = threadIdx.x, nt = blockDim.x;
-    if (a.rec_slot >= 0) {
+    if (rec_slot >= 0) {
         double sx = 0.0, sy = 0.0;
         for (int k = tid; k < b.nbct; k += nt) {
-            sx += b.fxy[2 * k];
-            sy += b.fxy[2 * k + 1];
+            sx += fxy[2 * k];
+            sy += fxy[2 * k + 1];
         }
         sx = block_sum(sx, sh);
         sy = block_sum(sy, sh);
-        double *rec = a.rec + (long long)a.rec_slot * a.rec_stride;
+        double *rec = a.rec + (long long)rec_slot * a.rec_stride;
         if (tid == 0) {
             rec[0] = sx;
             rec[1] = sy;
@@ -1239,22 +1254,27 @@ template <class S, class T> __device__ void wall_records(const StepArgs<S, T> &a
                 a.facc[3] += 1.0;
             }
         }
-        if (a.record_p) { // theta-sorted (:602-604); smoothed here unless a launch of its own does it
+        if (record_p) { // theta-sorted (:602-604); smoothed here unless a launch of its own does it
             double *ps = rec + 14;
-            for (int t = tid; t < b.nbct; t += nt) ps[t] = b.pcy[b.idx[t]];
-            if (!a.smooth_apart)
-                for (int t = tid >> 6; t < b.nbct; t += nt >> 6) smooth_output(a, rec, t);
+            for (int t = tid; t < b.nbct; t += nt) ps[t] = pcy[b.idx[t]];
+            if (!smooth_apart)
+                for (int t = tid >> 6; t < b.nbct; t += nt >> 6) smooth_output(a, pcy, rec, t);
         }
     }
-    if (a.inlet_prev) { // running mean of the inlet density of the previous step (:357-360)
+    if (inlet_prev) { // running mean of the inlet density of the previous step (:357-360)
         double s = 0.0;
-        for (int j = tid; j < a.ny; j += nt) s += a.inlet_prev[j];
+        for (int j = tid; j < a.ny; j += nt) s += inlet_prev[j];
         s = block_sum(s, sh);
         if (tid == 0) {
-            const double r = s / (double)a.ny, c = a.inlet_prev_c;
+            const double r = s / (double)a.ny, c = inlet_prev_c;
             *a.rho_inf_mean = ((c * *a.rho_inf_mean) + r) / (c + 1.0);
         }
+        __syncthreads(); // the next step's fold reads the mean
     }
+}
+template <class S, class T> __device__ void wall_records(const StepArgs<S, T> &a)
+{
+    wall_records_step(a, a.rec_slot, a.record_p, a.smooth_apart, a.b.fxy, a.b.pcy, a.inlet_prev, a.inlet_prev_c);
 }
 
 // All boundary nodes of one step in one workgroup: velocities (by dependency level), nodes.
@@ -1292,6 +1312,15 @@ template <class S, class T> __global__ void __launch_bounds__(64) wall_nodes_ker
     if (k < a.b.nbct) wall_node<S, T, true>(a, k);
 }
 
+// The records of the steps of a four-step pass in step order (one workgroup), after the pass's
+// side work: step k's forces and pressures from slot r.wslot[k] of the per-node arrays, and
+// the inlet-density fold of the step before it.
+template <class S, class T> __global__ void __launch_bounds__(256) wall_records4_kernel(StepArgs<S, T> a, RecSteps r)
+{
+    for (int k = 0; k < r.n; k++)
+        wall_records_step(a, r.rec_slot[k], r.record_p[k], 1, a.b.fxy + (long long)r.wslot[k] * 2 * a.b.nbct,
+                          a.b.pcy + (long long)r.wslot[k] * a.b.nbct, r.inlet_prev[k], r.inlet_prev_c[k]);
+}
 template <class S, class T> __global__ void __launch_bounds__(256) wall_records_kernel(StepArgs<S, T> a)
 {
     wall_records(a);
@@ -1832,12 +1861,22 @@ template <class S, class T> hipError_t launch_side(const StepArgs<S, T> &a, hipS
     else hipLaunchKernelGGL((side_kernel<S, T, false>), g, dim3(64), 0, st, a, (int)nband);
     return hipGetLastError();
 }
-template <class S, class T> hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st)
+template <class S, class T> hipError_t launch_records4(const StepArgs<S, T> &a, const RecSteps &r, hipStream_t st)
+{
+    if (r.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((wall_records4_kernel<S, T>), dim3(1), dim3(256), 0, st, a, r);
+    bool smooth = false;
+    for (int k = 0; k < r.n; k++) smooth = smooth || (r.record_p[k] && r.rec_slot[k] >= 0);
+    if (smooth)
+        hipLaunchKernelGGL((smooth4_kernel<S, T>), dim3(std::min(a.b.nbct, 1024), r.n), dim3(64), 0, st, a, r);
+    return hipGetLastError();
+}
+template <class S, class T> hipError_t launch_wall(const StepArgs<S, T> &a, hipStream_t st, bool records)
 {
     if (a.b.nlevels <= 1) {
         if (a.b.nbct > 0 && !a.wall_groups) // (else the side launch did the wall nodes)
             hipLaunchKernelGGL((wall_nodes_kernel<S, T>), dim3((a.b.nbct + 63) / 64), dim3(64), 0, st, a);
-        if (a.rec_slot >= 0 || a.inlet_prev) {
+        if (records && (a.rec_slot >= 0 || a.inlet_prev)) {
             StepArgs<S, T> r = a;
             r.smooth_apart = a.record_p && a.rec_slot >= 0 ? 1 : 0;
             hipLaunchKernelGGL((wall_records_kernel<S, T>), dim3(1), dim3(256), 0, st, r);
@@ -1950,7 +1989,8 @@ template <class S, class T> hipError_t launch_ghost_from_pops(const StepArgs<S, 
     template hipError_t launch_bulk2<S, T>(const StepArgs<S, T> &, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
     template hipError_t launch_bulkn<S, T>(const StepArgs<S, T> &, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
     template hipError_t launch_side<S, T>(const StepArgs<S, T> &, hipStream_t, bool);            \
-    template hipError_t launch_wall<S, T>(const StepArgs<S, T> &, hipStream_t);                  \
+    template hipError_t launch_wall<S, T>(const StepArgs<S, T> &, hipStream_t, bool);            \
+    template hipError_t launch_records4<S, T>(const StepArgs<S, T> &, const RecSteps &, hipStream_t); \
     template hipError_t launch_bulk_batch<S, T>(const StepArgs<S, T> *, const int2 *, int, int, int, hipStream_t); \
     template hipError_t launch_side_tiles<S, T>(const StepArgs<S, T> *, const int2 *, int, const int2 *, int, int, bool, \
                                                 bool, bool, hipStream_t);                            \
