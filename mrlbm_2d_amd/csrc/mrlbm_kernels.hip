@@ -100,10 +100,14 @@ template <class T> __device__ __forceinline__ T shdn(T v) { return dpp_shift_t<T
 
 // One row of the bulk march for the wave's 64 columns: rebuild the populations of the
 // row above (moments in mN), assemble the 9 pulled populations of row j, finish the
-// interior nodes of row j, and roll the register window up by one row.
-template <class S, class T, bool REC>
+// interior nodes of row j, and roll the register window up by one row.  PRE: the caller
+// loaded the node's four u-statistics accumulators (acc_pre) a row ahead, with the state's
+// row; loaded here, after the row's stores, the wave would wait for those stores and for
+// every load in flight before it could add the sample.
+template <class S, class T, bool REC, bool PRE = false>
 __device__ __forceinline__ void march_row(const StepArgs<S, T> &a, int c, bool valid, int j, const S mN[6],
-                                          T up_prev[3], T up_cur[3], T mid_cur[3], int cls)
+                                          T up_prev[3], T up_cur[3], T mid_cur[3], int cls,
+                                          const double *acc_pre = nullptr)
 {
     using F = typename FormOf<T>::F;
     T F9[Q];
@@ -138,10 +142,17 @@ __device__ __forceinline__ void march_row(const StepArgs<S, T> &a, int c, bool v
                         }
                 if (a.sample) {
                     const double ux = m[1], uy = m[2];
-                    a.acc[n] += ux;
-                    a.acc[a.plane + n] += uy;
-                    a.acc[2 * a.plane + n] += ux * ux;
-                    a.acc[3 * a.plane + n] += uy * uy;
+                    if constexpr (PRE) {
+                        a.acc[n] = acc_pre[0] + ux;
+                        a.acc[a.plane + n] = acc_pre[1] + uy;
+                        a.acc[2 * a.plane + n] = acc_pre[2] + ux * ux;
+                        a.acc[3 * a.plane + n] = acc_pre[3] + uy * uy;
+                    } else {
+                        a.acc[n] += ux;
+                        a.acc[a.plane + n] += uy;
+                        a.acc[2 * a.plane + n] += ux * ux;
+                        a.acc[3 * a.plane + n] += uy * uy;
+                    }
                 }
             }
             collide<T>(m, a.om, a.om1, a.incomp_bulk);
@@ -633,16 +644,19 @@ template <class S, class T, int NS, int H, bool REC> __device__ __forceinline__ 
 #pragma unroll
     for (int i = 0; i <= LA; i++) load_row(i, q[i]);
     T upp[NS + 1][3], upc[NS + 1][3], mid[NS + 1][3];
-    // REC: velocities of levels 1 .. NS-1, a ring of NS-s+1 entries per level s.  FP32 keeps
-    // them in registers (indices are compile-time once the march is unrolled; NS + 1 rows so
-    // that hu[l] stays in bounds where a loop over l = 1 .. NS names it); FP64, already at the
-    // register limit of two waves per SIMD, in LDS: per wave, level s at ring entries
-    // base(s) = sum over l < s of (NS-l+1), 2 values of 64 lanes each.
-    constexpr bool DL_LDS = REC && sizeof(S) == 8;
+    // REC: velocities of levels 1 .. NS-1 in LDS, a ring of NS-s+1 entries per level s: per
+    // wave, level s at ring entries base(s) = sum over l < s of (NS-l+1), 2 values of 64 lanes
+    // each (in registers they would cost FP32 its third wave per SIMD once the accumulators
+    // are loaded ahead, and FP64 its second).
     constexpr int DLN = NS * (NS + 1) / 2 - 1; // ring entries of levels 1 .. NS-1
-    T hu[NS + 1][NS + 1], hv[NS + 1][NS + 1];
-    __shared__ T dl[DL_LDS ? 4 : 1][DL_LDS ? DLN : 1][2][64];
+    __shared__ T dl[REC ? 4 : 1][REC ? DLN : 1][2][64];
     auto dl_at = [](int l, int e) { return (l - 1) * (NS + 1) - (l - 1) * l / 2 + e; }; // base(l) + e
+    // REC, FP32: the accumulators of the row the last level finishes in this march step are
+    // loaded at the step's start, before the next state row, and used at its end: the wave
+    // then waits for nothing younger than the previous step's stores.  Loaded after this
+    // step's stores, the wait would also take in those stores and the rows loaded ahead.
+    // FP64 has no registers to spare at two waves per SIMD and loads them at the end.
+    constexpr bool ACC_AHEAD = REC && sizeof(S) == 4;
 #pragma unroll
     for (int p = 0; p < NP; p++) {
         T Fin[Q]; // populations of the newest row of the level below
@@ -651,6 +665,15 @@ template <class S, class T, int NS, int H, bool REC> __device__ __forceinline__ 
         for (int i = 0; i < LA; i++)
 #pragma unroll
             for (int k = 0; k < 6; k++) q[i][k] = q[i + 1][k];
+        double av[4] = {};
+        if constexpr (ACC_AHEAD) {
+            const int j = j0 + p - 2 * NS;
+            if (p >= 2 * NS && a.qsample && outc && n_step_node<S, T, NS>(a, c, j)) {
+                const long long n = (long long)j * a.pitch + c;
+#pragma unroll
+                for (int k = 0; k < 4; k++) av[k] = a.acc[k * plane + n];
+            }
+        }
         if (p + LA + 1 < NP) load_row(p + LA + 1, q[LA]);
 #pragma unroll
         for (int s = 1; s <= NS; s++) {
@@ -676,13 +699,8 @@ template <class S, class T, int NS, int H, bool REC> __device__ __forceinline__ 
             F::moments(f, m);
             if constexpr (REC) {
                 if (s < NS) {
-                    if constexpr (DL_LDS) {
-                        dl[wave][dl_at(s, p % (NS - s + 1))][0][lane] = m[1];
-                        dl[wave][dl_at(s, p % (NS - s + 1))][1][lane] = m[2];
-                    } else {
-                        hu[s][p % (NS - s + 1)] = m[1];
-                        hv[s][p % (NS - s + 1)] = m[2];
-                    }
+                    dl[wave][dl_at(s, p % (NS - s + 1))][0][lane] = m[1];
+                    dl[wave][dl_at(s, p % (NS - s + 1))][1][lane] = m[2];
                 }
             }
             collide<T>(m, a.om, a.om1, a.incomp_bulk);
@@ -697,8 +715,13 @@ template <class S, class T, int NS, int H, bool REC> __device__ __forceinline__ 
                     for (int k = 0; k < 6; k++) buf_store<S>((S)m[k], rd[k], lb, ro);
                     if constexpr (REC) {
                         if (a.qsample) {
-                            double s0 = a.acc[n], s1 = a.acc[plane + n], s2 = a.acc[2 * plane + n],
-                                   s3 = a.acc[3 * plane + n];
+                            double s0, s1, s2, s3;
+                            if constexpr (ACC_AHEAD) {
+                                s0 = av[0]; s1 = av[1]; s2 = av[2]; s3 = av[3];
+                            } else {
+                                s0 = a.acc[n]; s1 = a.acc[plane + n]; s2 = a.acc[2 * plane + n];
+                                s3 = a.acc[3 * plane + n];
+                            }
 #pragma unroll
                             for (int l = 1; l <= NS; l++) {
                                 if (!((a.qsample >> (l - 1)) & 1)) continue;
@@ -706,13 +729,8 @@ template <class S, class T, int NS, int H, bool REC> __device__ __forceinline__ 
                                 const int e = (p - (NS - l)) % (NS - l + 1);
                                 double x = m[1], y = m[2];
                                 if (l < NS) {
-                                    if constexpr (DL_LDS) {
-                                        x = dl[wave][dl_at(l, e)][0][lane];
-                                        y = dl[wave][dl_at(l, e)][1][lane];
-                                    } else {
-                                        x = hu[l][e];
-                                        y = hv[l][e];
-                                    }
+                                    x = dl[wave][dl_at(l, e)][0][lane];
+                                    y = dl[wave][dl_at(l, e)][1][lane];
                                 }
                                 s0 += x;
                                 s1 += y;
@@ -779,6 +797,36 @@ template <class S, class T> __device__ void side_body(const StepArgs<S, T> &a, l
     if (a.has_boundary) stencil_moments_body(a, (int)t);
 }
 
+// Stand-ins read in place of the accumulators on steps that do not sample and of the node
+// class table on grids without a cylinder (never written).
+__device__ double acc_standin[4];
+__device__ unsigned char cls_standin[1];
+
+// The four u-statistics accumulators of node (c, j) (a marched row's, one row ahead of its
+// update; c and j lie in the grid), on steps that do not sample the stand-ins.  The loads
+// are issued either way: the compiler's waits count the loads issued after the one waited
+// for, and a load issued on one path only leaves a count that holds on every path (none),
+// so the wait for this row would take in the next row's loads too.
+template <class S, class T> __device__ __forceinline__ void load_acc_row(const StepArgs<S, T> &a, int c, int j, double v[4])
+{
+    const bool smp = a.sample;
+    const double *base = smp ? a.acc : acc_standin;
+    const long long n = smp ? (long long)j * a.pitch + c : 0, stride = smp ? a.plane : 1;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = base[k * stride + n];
+}
+
+// Class of node (c, j) from the cylinder's box table (0 outside it), the table read at a
+// clamped index whether or not the node lies in the box (see load_acc_row).
+template <class S, class T> __device__ __forceinline__ int box_class(const StepArgs<S, T> &a, int c, int j)
+{
+    const int bi = c - a.bx0, bj = j - a.by0;
+    const bool in = (unsigned)bi < (unsigned)a.bw && (unsigned)bj < (unsigned)a.bh;
+    const unsigned char *t = a.boxcls ? a.boxcls : cls_standin;
+    const int v = t[in ? bj * a.bw + bi : 0];
+    return in ? v : 0;
+}
+
 template <class S, class T> __device__ __forceinline__ int band_nodes(const StepArgs<S, T> &a)
 {
     return (int)band_items(a.rect, a.nrect, a.band_march);
@@ -789,7 +837,10 @@ template <class S, class T> __device__ __forceinline__ int band_nodes(const Step
 // columns x0+62g .. x0+62g+61 of rows y0 .. y1-1, lanes 0 and 63 carry the halo columns, and
 // each row is the bulk march's single-step row update (march_row) from rows loaded one
 // ahead.  Node classes come from the cylinder's box (boundary nodes are the wall's, solid
-// nodes are zeroed on plot steps); nodes of an earlier rectangle are left to it.
+// nodes are zeroed on plot steps); nodes of an earlier rectangle are left to it.  A row's
+// class byte and u-statistics accumulators are loaded with the moments a row ahead: the
+// wait for a value loaded just before its use also waits for every load issued before it,
+// the next row's among them.
 template <class S, class T> __device__ void band_march_segments(const StepArgs<S, T> &a, int q, int blk);
 template <class S, class T> __device__ void band_march(const StepArgs<S, T> &a, int blk)
 {
@@ -810,12 +861,10 @@ template <class S, class T> __device__ void band_march(const StepArgs<S, T> &a, 
     const int y0 = a.rect[q][1] + st * BAND_STRIP, y1 = min(y0 + BAND_STRIP, a.rect[q][1] + a.rect[q][3]);
     const int lane = threadIdx.x, c = x0 - 1 + 62 * g + lane, cl = min(c, a.nx - 1);
     const bool lanev = lane >= 1 && lane <= 62 && c < x0 + w; // rows y0-1 .. y1 lie in the grid
-    auto cls_of = [&](int j) {
-        const int bi = c - a.bx0, bj = j - a.by0;
-        return ((unsigned)bi < (unsigned)a.bw && (unsigned)bj < (unsigned)a.bh) ? (int)a.boxcls[bj * a.bw + bi] : 0;
-    };
+    auto cls_of = [&](int j) { return box_class(a, c, j); };
     auto valid = [&](int j) { return lanev && !in_rects(a, q, c, j); };
     S mA[6], mB[6];
+    double vA[4] = {}, vB[4] = {}; // accumulators of the row that mA / mB finish (sampling steps)
     T F9[Q], up_prev[3], up_cur[3], mid_cur[3];
     load_moments(a, cl, y0 - 1, mA);
     F::recon(mA, F9);
@@ -825,12 +874,18 @@ template <class S, class T> __device__ void band_march(const StepArgs<S, T> &a, 
     up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
     mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
     load_moments(a, cl, y0 + 1, mA);
-    for (int j = y0; j < y1; j += 2) {
-        if (j + 2 <= y1) load_moments(a, cl, j + 2, mB);
-        march_row<S, T, true>(a, c, valid(j), j, mA, up_prev, up_cur, mid_cur, cls_of(j));
+    load_acc_row(a, cl, y0, vA);
+    int kA = cls_of(y0), kB;
+    for (int j = y0; j < y1; j += 2) { // loads past row y1 read row y1 again (unused)
+        load_moments(a, cl, min(j + 2, y1), mB);
+        load_acc_row(a, cl, j + 1, vB);
+        kB = cls_of(j + 1);
+        march_row<S, T, true, true>(a, c, valid(j), j, mA, up_prev, up_cur, mid_cur, kA, vA);
         if (j + 1 >= y1) break;
-        if (j + 3 <= y1) load_moments(a, cl, j + 3, mA);
-        march_row<S, T, true>(a, c, valid(j + 1), j + 1, mB, up_prev, up_cur, mid_cur, cls_of(j + 1));
+        load_moments(a, cl, min(j + 3, y1), mA);
+        load_acc_row(a, cl, min(j + 2, y1), vA);
+        kA = cls_of(min(j + 2, y1));
+        march_row<S, T, true, true>(a, c, valid(j + 1), j + 1, mB, up_prev, up_cur, mid_cur, kB, vB);
     }
 }
 
@@ -852,12 +907,10 @@ template <class S, class T> __device__ void band_march_segments(const StepArgs<S
     const int y0 = yr + (8 * blk + seg) * R, y1 = min(y0 + R, yr + h); // rows y0 .. y1-1
     const bool lanev = i >= 1 && i <= w;
     auto row = [&](int j) { return min(j, a.ny - 1); }; // segments past the rectangle end
-    auto cls_of = [&](int j) {
-        const int bi = c - a.bx0, bj = j - a.by0;
-        return ((unsigned)bi < (unsigned)a.bw && (unsigned)bj < (unsigned)a.bh) ? (int)a.boxcls[bj * a.bw + bi] : 0;
-    };
+    auto cls_of = [&](int j) { return box_class(a, c, j); };
     auto valid = [&](int j) { return lanev && j < y1 && !in_rects(a, q, c, j); };
     S mA[6], mB[6];
+    double vA[4] = {}, vB[4] = {}; // accumulators of the row that mA / mB finish (sampling steps)
     T F9[Q], up_prev[3], up_cur[3], mid_cur[3];
     load_moments(a, cl, row(y0 - 1), mA);
     F::recon(mA, F9);
@@ -867,12 +920,18 @@ template <class S, class T> __device__ void band_march_segments(const StepArgs<S
     up_cur[0] = F9[2]; up_cur[1] = F9[5]; up_cur[2] = F9[6];
     mid_cur[0] = F9[0]; mid_cur[1] = F9[1]; mid_cur[2] = F9[3];
     load_moments(a, cl, row(y0 + 1), mA);
+    load_acc_row(a, cl, row(y0), vA);
+    int kA = cls_of(y0), kB;
     for (int k = 0; k < R; k += 2) { // every segment marches R rows (uniform trip count)
         const int j = y0 + k;
         load_moments(a, cl, row(j + 2), mB);
-        march_row<S, T, true>(a, c, valid(j), j, mA, up_prev, up_cur, mid_cur, cls_of(j));
-        if (k + 3 <= R) load_moments(a, cl, row(j + 3), mA);
-        march_row<S, T, true>(a, c, valid(j + 1), j + 1, mB, up_prev, up_cur, mid_cur, cls_of(j + 1));
+        load_acc_row(a, cl, row(j + 1), vB);
+        kB = cls_of(j + 1);
+        march_row<S, T, true, true>(a, c, valid(j), j, mA, up_prev, up_cur, mid_cur, kA, vA);
+        load_moments(a, cl, row(j + 3), mA); // the last one unused
+        load_acc_row(a, cl, row(j + 2), vA);
+        kA = cls_of(j + 2);
+        march_row<S, T, true, true>(a, c, valid(j + 1), j + 1, mB, up_prev, up_cur, mid_cur, kB, vB);
     }
 }
 
