@@ -1405,7 +1405,7 @@ template <class S, class T, bool REC, int H> __global__ void __launch_bounds__(2
 // block per tile; the side kernel takes the run from blockIdx.y, the boundary workgroup of
 // run r is block r.
 template <class S, class T, int H>
-__global__ void __launch_bounds__(256) bulk_batch_kernel(const StepArgs<S, T> *args, const int2 *tiles, int ntiles)
+__global__ void __launch_bounds__(256) bulk_batch_kernel(const StepArgs<S, T> *__restrict__ args, const int2 *__restrict__ tiles, int ntiles)
 {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 tl = tiles[t];
@@ -1415,7 +1415,7 @@ __global__ void __launch_bounds__(256) bulk_batch_kernel(const StepArgs<S, T> *a
 }
 
 // The wall of every run, one workgroup per run, velocities by dependency level.
-template <class S, class T> __global__ void __launch_bounds__(256) boundary_batch_kernel(const StepArgs<S, T> *args)
+template <class S, class T> __global__ void __launch_bounds__(256) boundary_batch_kernel(const StepArgs<S, T> *__restrict__ args)
 {
     const StepArgs<S, T> &a = args[blockIdx.x];
     if (!a.has_boundary) return;
@@ -1425,7 +1425,7 @@ template <class S, class T> __global__ void __launch_bounds__(256) boundary_batc
 // Side work of every run from a compact table of (run, block): wall groups, band marches, ring
 // column tiles, then blocks of 64 ring, band and stencil nodes, in single-wave workgroups,
 // none of them empty (side_block's order, per run).
-template <class S, class T> __global__ void __launch_bounds__(64) side_tiles_kernel(const StepArgs<S, T> *args, const int2 *tiles)
+template <class S, class T> __global__ void __launch_bounds__(64) side_tiles_kernel(const StepArgs<S, T> *__restrict__ args, const int2 *__restrict__ tiles)
 {
     const int2 t = tiles[blockIdx.x];
     const StepArgs<S, T> &a = args[t.x];
@@ -1434,7 +1434,7 @@ template <class S, class T> __global__ void __launch_bounds__(64) side_tiles_ker
 
 // Wall nodes of every run from a table of (run, block of 64 boundary nodes), for walls of one
 // dependency level; the records follow in wall_records_batch_kernel.
-template <class S, class T> __global__ void __launch_bounds__(64) wall_tiles_kernel(const StepArgs<S, T> *args, const int2 *tiles)
+template <class S, class T> __global__ void __launch_bounds__(64) wall_tiles_kernel(const StepArgs<S, T> *__restrict__ args, const int2 *__restrict__ tiles)
 {
     const int2 t = tiles[blockIdx.x];
     const StepArgs<S, T> &a = args[t.x];
@@ -1442,7 +1442,7 @@ template <class S, class T> __global__ void __launch_bounds__(64) wall_tiles_ker
     if (k < a.b.nbct) wall_node<S, T, true>(a, k);
 }
 
-template <class S, class T> __global__ void __launch_bounds__(256) wall_records_batch_kernel(const StepArgs<S, T> *args)
+template <class S, class T> __global__ void __launch_bounds__(256) wall_records_batch_kernel(const StepArgs<S, T> *__restrict__ args)
 {
     const StepArgs<S, T> &a = args[blockIdx.x];
     if (!a.has_boundary) return;
@@ -1450,7 +1450,7 @@ template <class S, class T> __global__ void __launch_bounds__(256) wall_records_
 }
 
 template <class S, class T, int H>
-__global__ void __launch_bounds__(256) bulk2_batch_kernel(const StepArgs<S, T> *args, const int2 *tiles, int ntiles)
+__global__ void __launch_bounds__(256) bulk2_batch_kernel(const StepArgs<S, T> *__restrict__ args, const int2 *__restrict__ tiles, int ntiles)
 {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 tl = tiles[t];
@@ -1461,7 +1461,7 @@ __global__ void __launch_bounds__(256) bulk2_batch_kernel(const StepArgs<S, T> *
 
 // Four steps per pass for every run from the flattened tile table (run, four-step block).
 template <class S, class T, int H>
-__global__ void __launch_bounds__(256, 2) bulkn_batch_kernel(const StepArgs<S, T> *args, const int2 *tiles, int ntiles)
+__global__ void __launch_bounds__(256, 2) bulkn_batch_kernel(const StepArgs<S, T> *__restrict__ args, const int2 *__restrict__ tiles, int ntiles)
 {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 tl = tiles[t];
