@@ -360,14 +360,23 @@ def stats_window_bench(steps=400, warmup=40):
 
 def copy_calibration(nbytes, reps):
     """A plain device copy of `nbytes` (torch's copy kernel), `reps` times: the known-size
-    transfer that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of a profiled run."""
+    transfer that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of a profiled run,
+    and the read + write rate a plain copy reaches (HIP events around the reps, after three
+    untimed copies)."""
     import torch
     x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(reps):
         y.copy_(x)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps, "ms_per_copy": round(ms, 5),
+                      "copy_gbs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
 
 
 def main():
